@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Throughput of the P3M Yukawa NVE loop on one MI355X.
+
+Workload (default): N = 1,000,000 unit-charge ions, box L = (4 pi N / 3)^(1/3),
+Gamma = 100 (T = 0.01), kappa = 2, 128^3 mesh, order-6 assignment, optimal
+Green function, uniform positions (RandomState(1)) and Maxwellian velocities
+(RandomState(2)), dt = 0.1, NVE Velocity-Verlet with per-step K, U, E, T and
+n(q, t) for the 20 smallest q along each axis.  alpha and rc are set for an
+RMS force error of at most 1e-5 relative to the mean force; the error against
+a tight FP64 reference (alpha 0.8, rc 8, 512^3) is measured at setup and
+reported.
+
+Timed region: `steps` fused NVE steps (kick-drift, binning, PP, assignment,
+FFT solve, gather + kick, FP64 reductions, n(q, t)) with every input already
+in HBM, bracketed by synchronisations; HIP events on the kernels' stream give
+the per-stage durations that feed the roofline entry.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "md-force-routines_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at RMS force err<=1e-5, chosen vs plain FP64 P3M, 1 GPU"
+PEAK_HBM = 8.0e12          # B/s, MI355X
+PEAK_FP64 = 78.6e12        # FLOP/s, MI355X dense FP64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--mesh", type=int, default=128)
+    ap.add_argument("--kappa", type=float, default=2.0)
+    ap.add_argument("--gamma", type=float, default=100.0)
+    ap.add_argument("--alpha", type=float, default=0.5)
+    ap.add_argument("--rc", type=float, default=5.5)
+    ap.add_argument("--dt", type=float, default=0.1)
+    ap.add_argument("--nq", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-err", action="store_true", help="skip the force-error measurement")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    return ap.parse_args()
+
+
+def stage_work(stage, N, M, rc):
+    """Algorithmic work of one launch of a stage: (bytes, flops).
+
+    Plain FP64: bin 60 B/particle; PP 56 B/particle and 61 FLOP per unique
+    in-cutoff pair (N rc^3 / 2 pairs at unit Wigner-Seitz density);
+    assignment 32 B/particle + 8 B/node; FFT solve 100 B/node (R2C 16,
+    Green x ik 36, three C2R 48); gather 56 B/particle + 24 B/node;
+    kick-drift 72 B/particle; n(q) 24 B/particle.
+    """
+    M3 = M ** 3
+    table = {
+        "bin": (60.0 * N, 0.0),
+        "pp": (56.0 * N, 61.0 * N * rc ** 3 / 2.0),
+        "assign": (32.0 * N + 8.0 * M3, 0.0),
+        "fft": (100.0 * M3, 0.0),
+        "gather": (56.0 * N + 24.0 * M3, 0.0),
+        "drift": (72.0 * N, 0.0),
+        "modes": (24.0 * N, 0.0),
+        "finalize": (0.0, 0.0),
+    }
+    return table[stage]
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mdforce.engine import P3M
+    from mdforce.initialize_pos_vel import initial_split
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N, M = a.n, a.mesh
+    L = (4.0 * math.pi * N / 3.0) ** (1.0 / 3.0)
+    T = 1.0 / a.gamma
+    pos0, vel0 = initial_split(1, 2, N, [L, L, L], T)
+
+    eng = P3M(N, M, L, a.kappa, a.alpha, a.rc, device=dev)
+    G_full, _ = eng.build_green(6, 3, want_full=True)
+    pos = torch.from_numpy(pos0).to(dev)
+    vel = torch.from_numpy(vel0).to(dev)
+    acc, en = eng.force(pos)
+    acc = acc.clone()
+    torch.cuda.synchronize()
+    acc0 = acc.cpu().numpy()
+
+    err = {}
+    if not a.no_err:
+        ref = P3M(N, 512, L, a.kappa, 0.8, 8.0, device=dev)
+        ref.build_green(6, 3)
+        acc_t, _ = ref.force(pos)
+        d = acc - acc_t
+        rms_d = float(torch.sqrt(torch.mean(torch.sum(d * d, 1))))
+        rms_f = float(torch.sqrt(torch.mean(torch.sum(acc_t * acc_t, 1))))
+        mean_f = float(torch.mean(torch.sqrt(torch.sum(acc_t * acc_t, 1))))
+        err = {"rms_rel_rms": rms_d / rms_f, "rms_rel_meanF": rms_d / mean_f,
+               "reference": "FP64 P3M alpha 0.8, rc 8, 512^3"}
+        ref.close()
+        del acc_t, d
+        torch.cuda.synchronize()
+
+    q = torch.arange(1, a.nq + 1, dtype=torch.float64, device=dev) * (2.0 * math.pi / L)
+    Z = None
+    # warm-up
+    if a.warmup:
+        eng.md_run(pos, vel, acc, a.warmup, a.dt, Z=Z, q=q)
+    energies = torch.empty((a.steps, 4), dtype=torch.float64, device=dev)
+    nqt = torch.empty((a.steps, a.nq, 3), dtype=torch.complex128, device=dev)
+    torch.cuda.synchronize()
+    eng.profile_start(events=16 * (a.steps + 4))
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    eng.md_run(pos, vel, acc, a.steps, a.dt, Z=Z, q=q, energies=energies, nqt=nqt)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(stop)
+    stages = eng.profile_read()
+    eng.profile_stop()
+    ms_per_step = gpu_ms / a.steps
+    value = N * a.steps / (gpu_ms / 1e3)
+
+    en_h = energies.cpu().numpy()
+    drift = float(np.max(np.abs(en_h[:, 0] - en_h[0, 0])) / (N * T))
+
+    # roofline of the dominant stage
+    dom = max((s for s in stages if s != "finalize"), key=lambda s: stages[s][0])
+    tot_ms, cnt = stages[dom]
+    avg_s = tot_ms / max(cnt, 1) / 1e3
+    by, fl = stage_work(dom, N, M, a.rc)
+    if fl > 0:
+        roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = {"pp": "k_pp_fp64", "fft": "fft passes", "gather": "k_gather", "assign": "k_assign",
+                      "bin": "bin kernels", "drift": "k_kick_drift", "modes": "k_modes"}.get(dom, dom)
+    roof["avg_launch_ms"] = avg_s * 1e3
+    stage_ms = {s: round(v[0] / max(v[1], 1), 4) for s, v in stages.items() if v[1]}
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle import c_oracle
+        c_oracle.build(ref=False)
+        Gh = G_full.cpu().numpy()
+        p_h, v_h = pos0.copy(), vel0.copy()
+        a_h = acc0.copy()  # the initial force (FP64, equal to the host's to 1e-12)
+        qh = q.cpu().numpy()
+        t0 = time.perf_counter()
+        c_oracle.md_steps(p_h, v_h, a_h, np.ones(N), a.cpu_steps, a.dt, L, a.kappa, a.alpha, a.rc, Gh, M, q=qh)
+        tc = time.perf_counter() - t0
+        cpu = {"value": N * a.cpu_steps / tc, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{a.cpu_steps} NVE step(s) of the same N={N}, {M}^3 workload, single-threaded C "
+                         f"restatement (oracle/c/p3m_oracle.c), {tc:.1f} s, host {platform.processor() or 'cpu'}"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP64 P3M Yukawa OCP NVE, N={N}, {M}^3 mesh (Gamma={a.gamma:g}, kappa={a.kappa:g})",
+            "form": "plain",
+            "N": N, "mesh": M, "L": L, "kappa": a.kappa, "Gamma": a.gamma, "alpha": a.alpha, "rc": a.rc,
+            "dt": a.dt, "nq": a.nq, "force_error": err, "energy_drift_per_NT": drift,
+            "stage_ms": stage_ms, "wall_s": wall, "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

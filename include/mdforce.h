@@ -1,0 +1,141 @@
+/*
+ * mdforce — MI355X (gfx950) P3M Yukawa force engine, C ABI.
+ *
+ * Every function returns 0 on success and a negative code on error; it never
+ * aborts.  mdf_last_error() returns a thread-local message for the last
+ * failure.  Array pointers are DEVICE pointers owned by the caller (e.g.
+ * torch tensors' data_ptr()); `stream` is a hipStream_t (NULL = default
+ * stream).  Calls are asynchronous on that stream.  A context owns its own
+ * scratch (cell lists, meshes, spectra, Green function) and allocates nothing
+ * after mdf_p3m_create.  One context per stream; contexts are not re-entrant.
+ *
+ * Layouts (the reference's, yukawa_MD_p3m_ord6.py:107-118):
+ *   pos, vel, acc : (n, 3) double, row-major (x, y, z per particle)
+ *   Z             : (n,) double charges
+ *   mesh          : (Mz, My, Mx) double, x fastest
+ *   G_k           : (Mz, My, Mx) double in fftshift order (yukawa_gf_opt.py:10,94)
+ */
+#ifndef MDFORCE_H
+#define MDFORCE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDF_ABI_VERSION 1
+
+/* Error codes. */
+#define MDF_OK 0
+#define MDF_EINVAL (-1)  /* bad argument */
+#define MDF_ENOMEM (-2)  /* device allocation failed */
+#define MDF_EHIP (-3)    /* HIP runtime error */
+#define MDF_ESTATE (-4)  /* call out of order (e.g. no Green function set) */
+
+/* Arithmetic of the force path. */
+#define MDF_PREC_FP64 0  /* plain: FP64 everywhere */
+#define MDF_PREC_MIXED 1 /* FP32 pair math / assignment / FFT, FP64 mesh sums and positions */
+
+typedef struct mdf_p3m_ctx mdf_p3m_ctx;
+
+typedef struct {
+    int n_max;        /* largest particle count passed to this context */
+    int mesh[3];      /* Mx, My, Mz (powers of two, 8..512) */
+    double box[3];    /* Lx, Ly, Lz */
+    double kappa;     /* Yukawa screening (kappa) */
+    double alpha;     /* Ewald splitting parameter (the reference's G / G_ew) */
+    double rc;        /* PP cutoff; needs rc <= L/3 per axis */
+    int order;        /* charge-assignment order; 6 is the only one supported */
+    int precision;    /* MDF_PREC_* */
+} mdf_p3m_params;
+
+/* Library identity. */
+int mdf_abi_version(void);
+const char *mdf_last_error(void);
+
+/* Context lifetime.  Allocates every device buffer the force path needs. */
+int mdf_p3m_create(const mdf_p3m_params *params, mdf_p3m_ctx **out);
+int mdf_p3m_destroy(mdf_p3m_ctx *ctx);
+
+/* Green function.
+ * set_green: upload G_k (device pointer, full (Mz,My,Mx) array in the
+ *   reference's fftshift order, as returned by gf_opt, yukawa_gf_opt.py:5-98).
+ * build_green: compute the same optimal influence function on the device
+ *   (yukawa_gf_opt.py:5-98) with assignment order p and alias ranges m_max;
+ *   if G_k_out is not NULL the full shifted array is also written there, and
+ *   if A_pm_out is not NULL the error sum A_pm (yukawa_gf_opt.py:96) is
+ *   written to that device double. */
+int mdf_p3m_set_green(mdf_p3m_ctx *ctx, const double *G_k, void *stream);
+int mdf_p3m_build_green(mdf_p3m_ctx *ctx, int p, int mx_max, int my_max, int mz_max,
+                        double *G_k_out, double *A_pm_out, void *stream);
+
+/* Full P3M force and energy: replaces force_pot (yukawa_p3m_ord6.py:6-29).
+ *   acc       <- af * (acc_pp + acc_pm)                       (n,3)
+ *   energies  <- [U, U_pp, U_pm, U_self] (device, 4 doubles), U = uf*(U_pp+U_pm+U_self),
+ *                U_self = -n*alpha/sqrt(pi) (yukawa_p3m_ord6.py:23-24)
+ * Optional mirrors of the reference's scratch side effects (NULL to skip):
+ *   acc_pp (n,3) = acc_s_r, acc_pm (n,3) = acc_fft, e_x/e_y/e_z (n,) = E_*_p,
+ *   rho (Mz,My,Mx) = rho_r. */
+int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, double af,
+                  double uf, double *acc, double *energies, double *acc_pp, double *acc_pm,
+                  double *e_x, double *e_y, double *e_z, double *rho, void *stream);
+
+/* Short-range part alone: replaces particle_particle (yukawa_pp.py:6).
+ * acc_pp <- PP acceleration (n,3) (unit charges, as the reference);
+ * u_pp   <- U_s_r (device double). */
+int mdf_pp_force(mdf_p3m_ctx *ctx, const double *pos, int n, double *acc_pp, double *u_pp,
+                 void *stream);
+
+/* Long-range part alone: replaces particle_mesh_fft_r (yukawa_pm_ord6.py:440).
+ * acc_pm <- Z*E_p (n,3); u_pm <- U_f (device double); e_x/e_y/e_z, rho optional. */
+int mdf_pm_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, double *acc_pm,
+                 double *u_pm, double *e_x, double *e_y, double *e_z, double *rho, void *stream);
+
+/* Velocity-Verlet pieces (velocity_verlet_p3m_ord6.py:5-26).
+ * kick_drift: vel += 0.5*acc*dt; pos += vel*dt; if pbc, single wrap per
+ *   component: pos > lmax -> pos - L, pos < lmin -> pos + L (:12-19).
+ * kick: vel += 0.5*acc*dt; k_sum (device double, may be NULL) <- sum(vel^2). */
+int mdf_verlet_kick_drift(mdf_p3m_ctx *ctx, double *pos, double *vel, const double *acc, int n,
+                          double dt, int pbc, const double *lmin, const double *lmax,
+                          void *stream);
+int mdf_verlet_kick(mdf_p3m_ctx *ctx, double *vel, const double *acc, int n, double dt,
+                    double *k_sum, void *stream);
+
+/* Velocity rescale (thermostat_p3m_ord6.py:9-23): from k_sum = sum(vel^2)
+ * (device), T = k_sum/n; fact = sqrt(T_d/T) if it <= 1999 else
+ * sqrt((20 T_d/T - 1)/20); vel *= fact.  t_out (device, may be NULL) <- T. */
+int mdf_thermostat_scale(mdf_p3m_ctx *ctx, double *vel, int n, const double *k_sum,
+                         double T_desired, int it, double *t_out, void *stream);
+
+/* Density modes n(q) = sum_j exp(-i q x_j) along x, y and z for nq wave
+ * numbers q (device, nq doubles): yukawa_MD_p3m_ord6.py:214-219.
+ * out: device complex (nq, 3) as interleaved (re, im) doubles. */
+int mdf_density_modes(mdf_p3m_ctx *ctx, const double *pos, int n, const double *q, int nq,
+                      double *out, void *stream);
+
+/* nsteps NVE Velocity-Verlet steps (the production loop of
+ * yukawa_MD_p3m_ord6.py:195-219) with kick/drift, force, kick, the FP64
+ * K/U/momentum reductions and n(q,t) fused into the step's kernels.
+ *   energies (device, nsteps*4): per step (E, K, U, T) with K = kf*sum(v^2),
+ *     T = K/(kf*n).
+ *   nqt (device complex, nsteps*nq*3) or NULL when nq == 0.
+ * pos/vel/acc are updated in place; acc must hold the force of pos on entry. */
+int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const double *Z, int n,
+               int nsteps, double dt, double af, double uf, double kf, const double *q, int nq,
+               double *energies, double *nqt, void *stream);
+
+/* Stage timing with HIP events recorded on the call's stream, for
+ * measurement only.  mode 1: start (nout = event pool size, 0 = default),
+ * mode 0: stop, mode 2: synchronise and write per-stage milliseconds
+ * out[0..7] and interval counts out[8..15] (nout >= 16) for the stages
+ * bin, pp, assign, fft, gather, drift, modes, finalize; resets the totals. */
+int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout);
+
+/* Diagnostics: number of PP tiles that needed more than one LDS pass and
+ * particles whose stencil fell outside their LDS brick (both handled, only
+ * slower), accumulated since create.  host pointer, 2 ints. */
+int mdf_p3m_stats(mdf_p3m_ctx *ctx, int *out2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDFORCE_H */

@@ -1,0 +1,512 @@
+// Context management and the extern "C" entry points of libmdforce.
+//
+// The entry points replace the reference's Python force API
+// (yukawa_p3m_ord6.force_pot, yukawa_p3m_ord6.py:6-29) and its children; the
+// Python layer (mdforce.yukawa_p3m_ord6 and friends) keeps the reference's
+// positional signatures and calls these through ctypes.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mdf_internal.h"
+
+namespace mdf {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int md_step(mdf_p3m_ctx *c, double *pos, double *vel, double *acc, const double *Z, int n, double dt, double af,
+            double uf, double kf, double *rec, hipStream_t s);
+int thermostat_scale(mdf_p3m_ctx *c, double *vel, int n, const double *k_sum, double T_desired, int it,
+                     double *t_out, hipStream_t s);
+
+static bool pow2(int m) { return m > 0 && (m & (m - 1)) == 0; }
+
+}  // namespace mdf
+
+// Stage timing: a pool of events recorded in stream order; the interval
+// between consecutive marks is charged to the stage of the first mark.
+struct Prof {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> stage;
+    size_t used = 0;
+    bool overflow = false;
+};
+
+namespace mdf {
+
+void prof_mark(mdf_p3m_ctx *c, int stage, hipStream_t s) {
+    Prof *p = c->prof;
+    if (!p) return;
+    if (p->used >= p->ev.size()) {
+        p->overflow = true;
+        return;
+    }
+    (void)hipEventRecord(p->ev[p->used], s);
+    p->stage[p->used] = stage;
+    ++p->used;
+}
+
+static void prof_free(mdf_p3m_ctx *c) {
+    if (!c->prof) return;
+    for (hipEvent_t e : c->prof->ev) (void)hipEventDestroy(e);
+    delete c->prof;
+    c->prof = nullptr;
+}
+
+template <class T>
+static int dmalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        *p = nullptr;
+        return MDF_ENOMEM;
+    }
+    return MDF_OK;
+}
+
+static void free_ctx(mdf_p3m_ctx *c) {
+    if (!c) return;
+    prof_free(c);
+    void *ptrs[] = {c->cell_count, c->cell_start, c->part_cell, c->part_slot, c->perm,    c->spos,
+                    c->acc_pp_s,   c->rho,        c->ck,        c->ek,        c->emesh,   c->green,
+                    c->tw[0],      c->tw[1],      c->tw[2],     c->part_pp,   c->part_pm, c->part_red,
+                    c->scal,       c->stats};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+static int check_n(mdf_p3m_ctx *c, int n) {
+    if (n < 0 || n > c->prm.n_max) {
+        set_error("particle count %d outside [0, n_max=%d]", n, c->prm.n_max);
+        return MDF_EINVAL;
+    }
+    return MDF_OK;
+}
+
+static int need_green(mdf_p3m_ctx *c) {
+    if (!c->have_green) {
+        set_error("no Green function: call mdf_p3m_set_green or mdf_p3m_build_green first");
+        return MDF_ESTATE;
+    }
+    return MDF_OK;
+}
+
+}  // namespace mdf
+
+using namespace mdf;
+
+extern "C" {
+
+int mdf_abi_version(void) { return MDF_ABI_VERSION; }
+
+const char *mdf_last_error(void) { return g_err; }
+
+int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
+    if (!p || !out) {
+        set_error("mdf_p3m_create: null argument");
+        return MDF_EINVAL;
+    }
+    *out = nullptr;
+    if (p->order != 6) {
+        set_error("charge-assignment order %d not supported (only 6, as yukawa_pm_ord6)", p->order);
+        return MDF_EINVAL;
+    }
+    if (p->precision != MDF_PREC_FP64) {
+        set_error("precision mode %d not available in this build", p->precision);
+        return MDF_EINVAL;
+    }
+    if (p->n_max <= 0) {
+        set_error("n_max must be positive");
+        return MDF_EINVAL;
+    }
+    if (!(p->alpha > 0.0) || !(p->rc > 0.0) || !(p->kappa >= 0.0)) {
+        set_error("need alpha > 0, rc > 0, kappa >= 0");
+        return MDF_EINVAL;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (!pow2(p->mesh[d]) || p->mesh[d] < 8 || p->mesh[d] > 512) {
+            set_error("mesh[%d] = %d: need a power of two in [8, 512]", d, p->mesh[d]);
+            return MDF_EINVAL;
+        }
+        if (!(p->box[d] > 0.0)) {
+            set_error("box[%d] must be positive", d);
+            return MDF_EINVAL;
+        }
+        if ((int)floor(p->box[d] / p->rc) < 3) {
+            set_error("rc = %g too large for box %g: the cell list needs L/rc >= 3 per axis", p->rc, p->box[d]);
+            return MDF_EINVAL;
+        }
+    }
+    mdf_p3m_ctx *c = new (std::nothrow) mdf_p3m_ctx();
+    if (!c) {
+        set_error("out of host memory");
+        return MDF_ENOMEM;
+    }
+    std::memset((void *)c, 0, sizeof(*c));
+    c->prm = *p;
+    Geom &g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = p->box[d];
+        g.M[d] = p->mesh[d];
+        g.h[d] = p->box[d] / p->mesh[d];
+        const int lxd = (int)floor(p->box[d] / p->rc);  // reference cells, yukawa_pp.py:45-47
+        g.nsub[d] = 2 * lxd;
+        g.nblk[d] = lxd;
+        g.w[d] = p->box[d] / g.nsub[d];
+    }
+    g.Mxh = g.M[0] / 2 + 1;
+    g.kappa = p->kappa;
+    g.alpha = p->alpha;
+    g.rc = p->rc;
+    c->ncell = g.nsub[0] * g.nsub[1] * g.nsub[2];
+    c->nblock = g.nblk[0] * g.nblk[1] * g.nblk[2];
+    if (c->nblock > kMaxPartials) {
+        delete c;
+        set_error("too many cells (%d > %d): increase rc", c->nblock, kMaxPartials);
+        return MDF_EINVAL;
+    }
+    c->nmesh = (size_t)g.M[0] * g.M[1] * g.M[2];
+    c->nspec = (size_t)g.M[2] * g.M[1] * g.Mxh;
+
+    // LDS tile capacity of the PP kernel: mean tile + 10 sigma (Poisson) + slack
+    const double mean_tile = 216.0 * (double)p->n_max / (double)c->ncell;
+    double tm = mean_tile + 10.0 * sqrt(mean_tile) + 128.0;
+    const double tcap = (150.0 * 1024.0 - 5120.0) / 24.0;
+    if (tm > tcap) tm = tcap;
+    if (tm < 512.0) tm = 512.0;
+    c->tile_max = (int)tm;
+    c->pp_lds = 24 * (size_t)c->tile_max + 4 * (224 + kPPWaves * kPPQueue);
+
+    // per-block mesh brick (assignment / gather)
+    for (int d = 0; d < 3; ++d) c->brick.B[d] = (int)ceil(2.0 * g.w[d] / g.h[d]) + 8;
+    const size_t bnodes = (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
+    // bricks too large for LDS fall back to global-memory atomics / reads
+    c->brick.lds_assign = (bnodes * 8 <= 96 * 1024) ? 1 : 0;
+    c->brick.lds_gather = (bnodes * 24 <= 72 * 1024) ? 1 : 0;
+
+    int r = MDF_OK;
+#define ALLOC(ptr, cnt) \
+    if (r == MDF_OK) r = dmalloc(&(ptr), (cnt))
+    ALLOC(c->cell_count, (size_t)c->ncell);
+    ALLOC(c->cell_start, (size_t)c->ncell + 1);
+    ALLOC(c->part_cell, (size_t)p->n_max);
+    ALLOC(c->part_slot, (size_t)p->n_max);
+    ALLOC(c->perm, (size_t)p->n_max);
+    ALLOC(c->spos, (size_t)p->n_max);
+    ALLOC(c->acc_pp_s, 3 * (size_t)p->n_max);
+    ALLOC(c->rho, c->nmesh);
+    ALLOC(c->ck, c->nspec);
+    ALLOC(c->ek, 3 * c->nspec);
+    ALLOC(c->emesh, 3 * c->nmesh);
+    ALLOC(c->green, c->nspec);
+    for (int d = 0; d < 3; ++d) ALLOC(c->tw[d], (size_t)g.M[d] / 2);
+    ALLOC(c->part_pp, (size_t)c->nblock);
+    ALLOC(c->part_pm, (size_t)g.M[1] * (g.Mxh + 3));
+    ALLOC(c->part_red, (size_t)kMaxPartials);
+    ALLOC(c->scal, (size_t)kScalCount);
+    ALLOC(c->stats, 4);
+#undef ALLOC
+    if (r != MDF_OK) {
+        free_ctx(c);
+        return r;
+    }
+    // twiddles exp(-2 pi i t / M), evaluated in long double
+    for (int d = 0; d < 3; ++d) {
+        const int m = g.M[d];
+        std::vector<double2> tw(m / 2);
+        for (int t = 0; t < m / 2; ++t) {
+            const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)m;
+            tw[t] = make_double2((double)cosl(ang), (double)-sinl(ang));
+        }
+        if (hipMemcpy(c->tw[d], tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("twiddle upload failed");
+            free_ctx(c);
+            return MDF_EHIP;
+        }
+    }
+    if ((r = fill_zero_i(c->cell_count, (size_t)c->ncell, nullptr)) != MDF_OK ||
+        (r = fill_zero_i(c->stats, 4, nullptr)) != MDF_OK || (r = fill_zero(c->scal, kScalCount, nullptr)) != MDF_OK ||
+        (r = fill_zero(c->green, c->nspec, nullptr)) != MDF_OK) {
+        free_ctx(c);
+        return r;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_error("device synchronisation failed after create");
+        free_ctx(c);
+        return MDF_EHIP;
+    }
+    *out = c;
+    return MDF_OK;
+}
+
+int mdf_p3m_destroy(mdf_p3m_ctx *ctx) {
+    if (!ctx) return MDF_OK;
+    (void)hipDeviceSynchronize();
+    free_ctx(ctx);
+    return MDF_OK;
+}
+
+int mdf_p3m_set_green(mdf_p3m_ctx *ctx, const double *G_k, void *stream) {
+    if (!ctx || !G_k) {
+        set_error("mdf_p3m_set_green: null argument");
+        return MDF_EINVAL;
+    }
+    return set_green(ctx, G_k, (hipStream_t)stream);
+}
+
+int mdf_p3m_build_green(mdf_p3m_ctx *ctx, int p, int mx_max, int my_max, int mz_max, double *G_k_out,
+                        double *A_pm_out, void *stream) {
+    if (!ctx) {
+        set_error("mdf_p3m_build_green: null context");
+        return MDF_EINVAL;
+    }
+    return build_green(ctx, p, mx_max, my_max, mz_max, G_k_out, A_pm_out, (hipStream_t)stream);
+}
+
+int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, double af, double uf, double *acc,
+                  double *energies, double *acc_pp, double *acc_pm, double *e_x, double *e_y, double *e_z,
+                  double *rho, void *stream) {
+    if (!ctx || (!pos && n > 0)) {
+        set_error("mdf_p3m_force: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    MDF_TRY(need_green(ctx));
+    hipStream_t s = (hipStream_t)stream;
+    prof_mark(ctx, kStageBin, s);
+    MDF_TRY(bin_particles(ctx, pos, Z, n, s));
+    prof_mark(ctx, kStagePP, s);
+    MDF_TRY(pp_forces(ctx, n, s));
+    prof_mark(ctx, kStageAssign, s);
+    MDF_TRY(pm_assign(ctx, n, rho, s));
+    prof_mark(ctx, kStageFFT, s);
+    MDF_TRY(pm_solve(ctx, s));
+    GatherOut o{};
+    o.acc = acc;
+    o.af = af;
+    o.include_pp = 1;
+    o.acc_pp = acc_pp;
+    o.acc_pm = acc_pm;
+    o.ex = e_x;
+    o.ey = e_y;
+    o.ez = e_z;
+    prof_mark(ctx, kStageGather, s);
+    MDF_TRY(pm_gather(ctx, n, o, s));
+    prof_mark(ctx, kStageFinal, s);
+    MDF_TRY(finalize_energy(ctx, n, uf, 1, 1, energies, s));
+    prof_mark(ctx, kStageEnd, s);
+    return MDF_OK;
+}
+
+int mdf_pp_force(mdf_p3m_ctx *ctx, const double *pos, int n, double *acc_pp, double *u_pp, void *stream) {
+    if (!ctx || (!pos && n > 0)) {
+        set_error("mdf_pp_force: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    hipStream_t s = (hipStream_t)stream;
+    MDF_TRY(bin_particles(ctx, pos, nullptr, n, s));
+    MDF_TRY(pp_forces(ctx, n, s));
+    if (acc_pp) MDF_TRY(unpermute3(ctx, ctx->acc_pp_s, acc_pp, n, s));
+    MDF_TRY(finalize_energy(ctx, n, 1.0, 1, 0, nullptr, s));
+    if (u_pp) MDF_HIP(hipMemcpyAsync(u_pp, ctx->scal + kScalUpp, sizeof(double), hipMemcpyDeviceToDevice, s));
+    return MDF_OK;
+}
+
+int mdf_pm_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, double *acc_pm, double *u_pm,
+                 double *e_x, double *e_y, double *e_z, double *rho, void *stream) {
+    if (!ctx || (!pos && n > 0)) {
+        set_error("mdf_pm_force: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    MDF_TRY(need_green(ctx));
+    hipStream_t s = (hipStream_t)stream;
+    MDF_TRY(bin_particles(ctx, pos, Z, n, s));
+    MDF_TRY(pm_assign(ctx, n, rho, s));
+    MDF_TRY(pm_solve(ctx, s));
+    GatherOut o{};
+    o.acc = acc_pm;
+    o.af = 1.0;
+    o.include_pp = 0;
+    o.ex = e_x;
+    o.ey = e_y;
+    o.ez = e_z;
+    MDF_TRY(pm_gather(ctx, n, o, s));
+    MDF_TRY(finalize_energy(ctx, n, 1.0, 0, 1, nullptr, s));
+    if (u_pm) MDF_HIP(hipMemcpyAsync(u_pm, ctx->scal + kScalUpm, sizeof(double), hipMemcpyDeviceToDevice, s));
+    return MDF_OK;
+}
+
+int mdf_verlet_kick_drift(mdf_p3m_ctx *ctx, double *pos, double *vel, const double *acc, int n, double dt, int pbc,
+                          const double *lmin, const double *lmax, void *stream) {
+    if (!ctx || (n > 0 && (!pos || !vel || !acc))) {
+        set_error("mdf_verlet_kick_drift: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    return kick_drift(ctx, pos, vel, acc, n, dt, pbc, lmin, lmax, nullptr, (hipStream_t)stream);
+}
+
+int mdf_verlet_kick(mdf_p3m_ctx *ctx, double *vel, const double *acc, int n, double dt, double *k_sum,
+                    void *stream) {
+    if (!ctx || (n > 0 && (!vel || !acc))) {
+        set_error("mdf_verlet_kick: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    return kick(ctx, vel, acc, n, dt, k_sum, (hipStream_t)stream);
+}
+
+int mdf_thermostat_scale(mdf_p3m_ctx *ctx, double *vel, int n, const double *k_sum, double T_desired, int it,
+                         double *t_out, void *stream) {
+    if (!ctx || !k_sum || (n > 0 && !vel)) {
+        set_error("mdf_thermostat_scale: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    return thermostat_scale(ctx, vel, n, k_sum, T_desired, it, t_out, (hipStream_t)stream);
+}
+
+int mdf_density_modes(mdf_p3m_ctx *ctx, const double *pos, int n, const double *q, int nq, double *out,
+                      void *stream) {
+    if (!ctx || (nq > 0 && (!q || !out)) || (n > 0 && !pos)) {
+        set_error("mdf_density_modes: null argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    return density_modes(ctx, pos, n, q, nq, out, (hipStream_t)stream);
+}
+
+int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const double *Z, int n, int nsteps, double dt,
+               double af, double uf, double kf, const double *q, int nq, double *energies, double *nqt,
+               void *stream) {
+    if (!ctx || (n > 0 && (!pos || !vel || !acc)) || nsteps < 0 || (nq > 0 && (!q || !nqt))) {
+        set_error("mdf_md_run: bad argument");
+        return MDF_EINVAL;
+    }
+    MDF_TRY(check_n(ctx, n));
+    MDF_TRY(need_green(ctx));
+    hipStream_t s = (hipStream_t)stream;
+    for (int it = 0; it < nsteps; ++it) {
+        MDF_TRY(md_step(ctx, pos, vel, acc, Z, n, dt, af, uf, kf, energies ? energies + 4 * (size_t)it : nullptr, s));
+        if (nq > 0) {
+            prof_mark(ctx, kStageModes, s);
+            MDF_TRY(density_modes(ctx, pos, n, q, nq, nqt + (size_t)it * nq * 6, s));
+        }
+        prof_mark(ctx, kStageEnd, s);
+    }
+    return MDF_OK;
+}
+
+int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout) {
+    if (!ctx) {
+        set_error("mdf_p3m_profile: null context");
+        return MDF_EINVAL;
+    }
+    if (mode == 0) {
+        prof_free(ctx);
+        return MDF_OK;
+    }
+    if (mode == 1) {
+        prof_free(ctx);
+        Prof *p = new (std::nothrow) Prof();
+        if (!p) {
+            set_error("out of host memory");
+            return MDF_ENOMEM;
+        }
+        const size_t cap = nout > 0 ? (size_t)nout : 65536;
+        p->ev.resize(cap);
+        p->stage.resize(cap);
+        for (size_t i = 0; i < cap; ++i) {
+            if (hipEventCreate(&p->ev[i]) != hipSuccess) {
+                p->ev.resize(i);
+                p->stage.resize(i);
+                ctx->prof = p;
+                prof_free(ctx);
+                set_error("hipEventCreate failed");
+                return MDF_EHIP;
+            }
+        }
+        ctx->prof = p;
+        return MDF_OK;
+    }
+    if (mode == 2) {
+        Prof *p = ctx->prof;
+        if (!p || !out || nout < 2 * kStageCount) {
+            set_error("mdf_p3m_profile: profiling off or output too small (need %d doubles)", 2 * kStageCount);
+            return MDF_EINVAL;
+        }
+        for (int i = 0; i < 2 * kStageCount; ++i) out[i] = 0.0;
+        if (p->used) MDF_HIP(hipEventSynchronize(p->ev[p->used - 1]));
+        for (size_t i = 0; i + 1 < p->used; ++i) {
+            const int st = p->stage[i];
+            if (st < 0 || st >= kStageCount) continue;
+            float ms = 0.f;
+            MDF_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+            out[st] += ms;
+            out[kStageCount + st] += 1.0;
+        }
+        p->used = 0;
+        if (p->overflow) {
+            p->overflow = false;
+            set_error("profile event pool overflowed; totals are partial");
+            return MDF_ESTATE;
+        }
+        return MDF_OK;
+    }
+    set_error("mdf_p3m_profile: mode must be 0, 1 or 2");
+    return MDF_EINVAL;
+}
+
+int mdf_p3m_stats(mdf_p3m_ctx *ctx, int *out2) {
+    if (!ctx || !out2) {
+        set_error("mdf_p3m_stats: null argument");
+        return MDF_EINVAL;
+    }
+    int h[4];
+    MDF_HIP(hipMemcpy(h, ctx->stats, sizeof h, hipMemcpyDeviceToHost));
+    out2[0] = h[0];
+    out2[1] = h[1];
+    if (h[2]) {
+        set_error("a PP tile row exceeded the LDS tile capacity; forces of that step are incomplete");
+        return MDF_ESTATE;
+    }
+    return MDF_OK;
+}
+
+}  // extern "C"
+
+namespace mdf {
+
+__global__ void k_unpermute3(const double *__restrict__ src, const int *__restrict__ perm, double *__restrict__ dst,
+                             int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = perm[s];
+    dst[3 * (size_t)i + 0] = src[3 * (size_t)s + 0];
+    dst[3 * (size_t)i + 1] = src[3 * (size_t)s + 1];
+    dst[3 * (size_t)i + 2] = src[3 * (size_t)s + 2];
+}
+
+int unpermute3(mdf_p3m_ctx *c, const double *src_sorted, double *dst, int n, hipStream_t s) {
+    if (n <= 0) return MDF_OK;
+    hipLaunchKernelGGL(k_unpermute3, dim3((n + 255) / 256), dim3(256), 0, s, src_sorted, c->perm, dst, n);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+}  // namespace mdf

@@ -1,0 +1,302 @@
+// Integrator, thermostat, FP64 reductions and density modes.
+//
+// References:
+//   kick / drift / wrap   velocity_verlet_p3m_ord6.py:5-26
+//   velocity rescale      thermostat_p3m_ord6.py:5-25
+//   K, E, T               yukawa_MD_p3m_ord6.py:199-201
+//   n(q)                  yukawa_MD_p3m_ord6.py:214-219
+//   U assembly            yukawa_p3m_ord6.py:21-24
+// Arithmetic follows the reference expression by expression (v + 0.5*a*dt,
+// x + v*dt, the strict > / < single wrap), so positions and velocities agree
+// bit for bit given the same accelerations.  All reductions are FP64 over
+// per-block partials in a fixed order (deterministic).
+#include "mdf_internal.h"
+
+namespace mdf {
+
+__device__ __forceinline__ double block_sum_256(double v, double *red4) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = red4[0] + red4[1] + red4[2] + red4[3];
+    __syncthreads();
+    return r;
+}
+
+struct WrapBox {
+    double L[3], lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(256) void k_kick_drift(double *__restrict__ pos, double *__restrict__ vel,
+                                                    const double *__restrict__ acc, int n3, double dt, int pbc,
+                                                    WrapBox wb, const double *__restrict__ scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const int d = i % 3;
+    double v = vel[i];
+    if (scale) v = v * (*scale);
+    v = v + 0.5 * acc[i] * dt;
+    double x = pos[i] + v * dt;
+    if (pbc) {
+        if (x > wb.hi[d]) x = x - wb.L[d];
+        if (x < wb.lo[d]) x = x + wb.L[d];
+    }
+    vel[i] = v;
+    pos[i] = x;
+}
+
+__global__ __launch_bounds__(256) void k_kick(double *__restrict__ vel, const double *__restrict__ acc, int n3,
+                                              double dt, double *__restrict__ part) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += gridDim.x * blockDim.x) {
+        const double v = vel[i] + 0.5 * acc[i] * dt;
+        vel[i] = v;
+        s += v * v;
+    }
+    s = block_sum_256(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    s = block_sum_256(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// thermostat_p3m_ord6.py:9-23 with K = kf*sum v^2 and T = K/kf/N = sum v^2 / N
+__device__ __forceinline__ double rescale_factor(double vsq, int n, double T_desired, int it, double *T_out) {
+    const double T = vsq / (double)n;
+    if (T_out) *T_out = T;
+    return it <= 1999 ? sqrt(T_desired / T) : sqrt((20.0 * T_desired / T - 1.0) / 20.0);
+}
+
+__global__ __launch_bounds__(256) void k_rescale(double *__restrict__ vel, int n3, int n, const double *__restrict__ k_sum,
+                                                 double T_desired, int it, double *__restrict__ T_out,
+                                                 double *__restrict__ fact_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double f = rescale_factor(*k_sum, n, T_desired, it, nullptr);
+    if (i == 0) {
+        rescale_factor(*k_sum, n, T_desired, it, T_out);
+        if (fact_out) *fact_out = f;
+    }
+    if (i < n3) vel[i] = vel[i] * f;
+}
+
+// Density modes: block (b, iq) sums cos and sin of q x_a over its particles.
+__global__ __launch_bounds__(256) void k_modes(const double *__restrict__ pos, int n, const double *__restrict__ q,
+                                               double *__restrict__ part) {
+    __shared__ double red[4];
+    const double qq = q[blockIdx.y];
+    double c[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double sv, cv;
+            sincos(-qq * pos[3 * j + a], &sv, &cv);
+            c[a] += cv;
+            sn[a] += sv;
+        }
+    }
+    double *dst = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double cs = block_sum_256(c[a], red);
+        const double ss = block_sum_256(sn[a], red);
+        if (threadIdx.x == 0) {
+            dst[2 * a] = cs;
+            dst[2 * a + 1] = ss;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_modes_final(const double *__restrict__ part, int nblk, int nq,
+                                                    double *__restrict__ out) {
+    const int iq = blockIdx.x;
+    const int c = threadIdx.x;  // 0..5
+    if (c >= 6) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[((size_t)iq * nblk + b) * 6 + c];
+    out[(size_t)iq * 6 + c] = s;  // (re, im) pairs for the three axes
+}
+
+// U = uf (U_pp + U_pm + U_self) with U_pm = 1/2 sum |rho_k|^2 G / V; optional
+// K = kf sum v^2 and a per-step record (E, K, U, T).
+struct FinalArgs {
+    const double *pp;
+    int npp;
+    const double *pm;
+    int npm;
+    const double *kp;
+    int nk;
+    double half_inv_V, u_self, uf, kf;
+    int n;
+    double *scal;
+    double *energies;  // 4 or null
+    double *rec;       // 4 or null
+};
+
+__global__ __launch_bounds__(256) void k_finalize(FinalArgs P) {
+    __shared__ double red[4];
+    double a = 0.0, b = 0.0, k = 0.0;
+    for (int i = threadIdx.x; i < P.npp; i += blockDim.x) a += P.pp[i];
+    for (int i = threadIdx.x; i < P.npm; i += blockDim.x) b += P.pm[i];
+    for (int i = threadIdx.x; i < P.nk; i += blockDim.x) k += P.kp[i];
+    a = block_sum_256(a, red);
+    b = block_sum_256(b, red);
+    k = block_sum_256(k, red);
+    if (threadIdx.x == 0) {
+        const double Upp = a;
+        const double Upm = b * P.half_inv_V;
+        const double U = P.uf * (Upp + Upm + P.u_self);
+        P.scal[kScalU] = U;
+        P.scal[kScalUpp] = Upp;
+        P.scal[kScalUpm] = Upm;
+        P.scal[kScalUself] = P.u_self;
+        if (P.energies) {
+            P.energies[0] = U;
+            P.energies[1] = Upp;
+            P.energies[2] = Upm;
+            P.energies[3] = P.u_self;
+        }
+        if (P.nk) {
+            const double K = P.kf * k;
+            P.scal[kScalK] = K;
+            P.scal[kScalT] = K / P.kf / (double)P.n;
+            if (P.rec) {
+                P.rec[0] = K + U;
+                P.rec[1] = K;
+                P.rec[2] = U;
+                P.rec[3] = K / P.kf / (double)P.n;
+            }
+        }
+    }
+}
+
+int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, double *energies, hipStream_t s) {
+    FinalArgs P{};
+    P.pp = c->part_pp;
+    P.npp = have_pp ? c->nblock : 0;
+    P.pm = c->part_pm;
+    P.npm = have_pm ? c->n_part_pm : 0;
+    P.kp = nullptr;
+    P.nk = 0;
+    P.half_inv_V = 0.5 / (c->g.L[0] * c->g.L[1] * c->g.L[2]);
+    P.u_self = (have_pp && have_pm) ? -(double)n * c->g.alpha / sqrt(M_PI) : 0.0;
+    P.uf = uf;
+    P.kf = 1.5;
+    P.n = n;
+    P.scal = c->scal;
+    P.energies = energies;
+    P.rec = nullptr;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, P);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+static int finalize_step(mdf_p3m_ctx *c, int n, double uf, double kf, double *rec, hipStream_t s) {
+    FinalArgs P{};
+    P.pp = c->part_pp;
+    P.npp = c->nblock;
+    P.pm = c->part_pm;
+    P.npm = c->n_part_pm;
+    P.kp = c->part_red;
+    P.nk = c->nblock;
+    P.half_inv_V = 0.5 / (c->g.L[0] * c->g.L[1] * c->g.L[2]);
+    P.u_self = -(double)n * c->g.alpha / sqrt(M_PI);
+    P.uf = uf;
+    P.kf = kf;
+    P.n = n;
+    P.scal = c->scal;
+    P.energies = nullptr;
+    P.rec = rec;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, P);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+int kick_drift(mdf_p3m_ctx *c, double *pos, double *vel, const double *acc, int n, double dt, int pbc,
+               const double lmin[3], const double lmax[3], const double *scale, hipStream_t s) {
+    (void)c;
+    if (n <= 0) return MDF_OK;
+    WrapBox wb;
+    for (int d = 0; d < 3; ++d) {
+        wb.L[d] = c->g.L[d];
+        wb.lo[d] = lmin ? lmin[d] : 0.0;
+        wb.hi[d] = lmax ? lmax[d] : c->g.L[d];
+    }
+    const int n3 = 3 * n;
+    hipLaunchKernelGGL(k_kick_drift, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, vel, acc, n3, dt, pbc, wb, scale);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+int kick(mdf_p3m_ctx *c, double *vel, const double *acc, int n, double dt, double *k_sum, hipStream_t s) {
+    if (n <= 0) return MDF_OK;
+    const int n3 = 3 * n;
+    int nb = (n3 + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_kick, dim3(nb), dim3(256), 0, s, vel, acc, n3, dt, c->part_red);
+    MDF_LAUNCHED();
+    if (k_sum) {
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, c->part_red, nb, k_sum);
+        MDF_LAUNCHED();
+    }
+    return MDF_OK;
+}
+
+int density_modes(mdf_p3m_ctx *c, const double *pos, int n, const double *q, int nq, double *out, hipStream_t s) {
+    if (nq <= 0) return MDF_OK;
+    int nb = (n + 255) / 256;
+    if (nb > 64) nb = 64;
+    if (nb < 1) nb = 1;
+    if ((size_t)nb * nq * 6 > (size_t)kMaxPartials) {
+        set_error("density_modes: too many wave numbers (%d)", nq);
+        return MDF_EINVAL;
+    }
+    hipLaunchKernelGGL(k_modes, dim3(nb, nq), dim3(256), 0, s, pos, n, q, c->part_red);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL(k_modes_final, dim3(nq), dim3(64), 0, s, c->part_red, nb, nq, out);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+int thermostat_scale(mdf_p3m_ctx *c, double *vel, int n, const double *k_sum, double T_desired, int it,
+                     double *t_out, hipStream_t s) {
+    if (n <= 0) return MDF_OK;
+    const int n3 = 3 * n;
+    hipLaunchKernelGGL(k_rescale, dim3((n3 + 255) / 256), dim3(256), 0, s, vel, n3, n, k_sum, T_desired, it, t_out,
+                       c->scal + kScalFact);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+// One NVE step on device data (no host synchronisation).
+int md_step(mdf_p3m_ctx *c, double *pos, double *vel, double *acc, const double *Z, int n, double dt, double af,
+            double uf, double kf, double *rec, hipStream_t s) {
+    prof_mark(c, kStageDrift, s);
+    MDF_TRY(kick_drift(c, pos, vel, acc, n, dt, 1, nullptr, nullptr, nullptr, s));
+    prof_mark(c, kStageBin, s);
+    MDF_TRY(bin_particles(c, pos, Z, n, s));
+    prof_mark(c, kStagePP, s);
+    MDF_TRY(pp_forces(c, n, s));
+    prof_mark(c, kStageAssign, s);
+    MDF_TRY(pm_assign(c, n, nullptr, s));
+    prof_mark(c, kStageFFT, s);
+    MDF_TRY(pm_solve(c, s));
+    GatherOut o{};
+    o.acc = acc;
+    o.af = af;
+    o.include_pp = 1;
+    o.vel = vel;
+    o.kick_dt = dt;
+    prof_mark(c, kStageGather, s);
+    MDF_TRY(pm_gather(c, n, o, s));
+    prof_mark(c, kStageFinal, s);
+    MDF_TRY(finalize_step(c, n, uf, kf, rec, s));
+    return MDF_OK;
+}
+
+}  // namespace mdf

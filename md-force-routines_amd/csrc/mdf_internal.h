@@ -1,0 +1,190 @@
+// Internal declarations shared by the mdforce HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/mdforce.h"
+
+namespace mdf {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const char *fmt, ...);
+
+#define MDF_HIP(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            ::mdf::set_error("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                \
+            return MDF_EHIP;                                                           \
+        }                                                                              \
+    } while (0)
+
+#define MDF_LAUNCHED()                                                                 \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess) {                                                        \
+            ::mdf::set_error("kernel launch: %s (%s:%d)", hipGetErrorString(e_),       \
+                             __FILE__, __LINE__);                                      \
+            return MDF_EHIP;                                                           \
+        }                                                                              \
+    } while (0)
+
+#define MDF_TRY(expr)            \
+    do {                         \
+        int r_ = (expr);         \
+        if (r_ != MDF_OK) return r_; \
+    } while (0)
+
+// ------------------------------------------------------------- geometry --
+// Box, mesh and cell geometry passed by value to kernels.
+struct Geom {
+    double L[3];      // box
+    double h[3];      // mesh spacing L/M
+    double w[3];      // PP sub-cell width L/nsub
+    int M[3];         // mesh points
+    int Mxh;          // M[0]/2 + 1 (half-spectrum length along x)
+    int nsub[3];      // PP sub-cells per axis (even; two per reference cell)
+    int nblk[3];      // PP blocks (2x2x2 sub-cells = one reference cell)
+    double kappa, alpha, rc;
+};
+
+// Per-block LDS mesh brick for assignment / gather.
+struct Brick {
+    int B[3];         // brick extent in mesh nodes
+    int lds_assign;   // 1 if the charge brick fits LDS for the assignment
+    int lds_gather;   // 1 if the 3-field brick fits LDS for the gather
+};
+
+constexpr int kPPThreads = 256;     // 4 waves share one 6x6x6 sub-cell tile
+constexpr int kPPWaves = kPPThreads / 64;
+constexpr int kPPQueue = 128;       // per-wave in-cutoff pair queue
+constexpr int kPMThreads = 256;
+constexpr int kMaxPartials = 1 << 20;
+
+// ctx->stats (device ints): [0] PP tiles needing more than one LDS pass,
+// [1] particles outside their assignment/gather brick (handled through
+// global memory), [2] error flag: a tile row exceeded the LDS capacity.
+
+}  // namespace mdf
+
+struct mdf_p3m_ctx {
+    mdf_p3m_params prm;
+    mdf::Geom g;
+    mdf::Brick brick;
+    int ncell;          // total sub-cells
+    int nblock;         // total PP blocks
+    int tile_max;       // LDS tile capacity (particles)
+    size_t pp_lds;      // dynamic LDS bytes of the PP kernel
+    size_t nmesh;       // Mx*My*Mz
+    size_t nspec;       // Mz*My*Mxh
+    bool have_green;
+
+    // binning
+    int *cell_count;    // ncell, zero between steps
+    int *cell_start;    // ncell + 1
+    int *part_cell;     // n_max
+    int *part_slot;     // n_max
+    int *perm;          // sorted -> caller index
+    double4 *spos;      // sorted (x, y, z, Z), positions wrapped into [0, L]
+    double *acc_pp_s;   // sorted PP acceleration (n, 3)
+
+    // mesh
+    double *rho;        // nmesh
+    double2 *ck;        // nspec
+    double2 *ek;        // 3 * nspec
+    double *emesh;      // 3 * nmesh, interleaved (Ex, Ey, Ez) per node
+    double *green;      // nspec, natural order half spectrum
+    double2 *tw[3];     // twiddles per axis, M/2 entries
+
+    // reductions
+    double *part_pp;    // per-block PP energy partials (nblock)
+    double *part_pm;    // per-block PM energy partials
+    int n_part_pm;
+    double *part_red;   // generic partials (kMaxPartials doubles)
+    double *scal;       // device scalars, see kScal*
+    int *stats;         // device Stats
+    struct Prof *prof;  // optional stage timing (null when off)
+};
+
+namespace mdf {
+
+// Device scalar slots in ctx->scal.
+enum {
+    kScalU = 0, kScalUpp, kScalUpm, kScalUself,  // energies (U first, as returned)
+    kScalK, kScalT, kScalFact, kScalPad,
+    kScalCount = 16
+};
+
+// Stage ids of the optional event timing (mdf_p3m_profile).
+enum {
+    kStageBin = 0, kStagePP, kStageAssign, kStageFFT, kStageGather, kStageDrift, kStageModes,
+    kStageFinal, kStageCount, kStageEnd = -1
+};
+// Records an event on s opening `stage` (or closing the previous one with
+// kStageEnd); a no-op unless profiling is on.
+void prof_mark(mdf_p3m_ctx *c, int stage, hipStream_t s);
+
+// ----------------------------------------------------------- launchers --
+int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s);
+int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s);
+int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s);
+int pm_solve(mdf_p3m_ctx *c, hipStream_t s);
+// Gather plus epilogue: acc (caller order) = af*(acc_pp_s + Z*E_p); optional
+// mirrors; if kick_dt != 0 also vel += 0.5*acc*kick_dt with K partials.
+struct GatherOut {
+    double *acc;      // (n,3) or null
+    double af;
+    int include_pp;   // add acc_pp_s into acc
+    double *acc_pp;   // mirror (n,3) or null
+    double *acc_pm;   // mirror (n,3) or null
+    double *ex, *ey, *ez;
+    double *vel;      // for the fused second half-kick, or null
+    double kick_dt;
+};
+int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s);
+int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, double *energies,
+                    hipStream_t s);
+int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, double *A_pm,
+                hipStream_t s);
+int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
+
+// Fill helpers (kernels, so they can live inside captured graphs).
+int fill_zero(double *p, size_t n, hipStream_t s);
+int fill_zero_i(int *p, size_t n, hipStream_t s);
+
+// Integrator pieces.
+int kick_drift(mdf_p3m_ctx *c, double *pos, double *vel, const double *acc, int n, double dt,
+               int pbc, const double lmin[3], const double lmax[3], const double *scale,
+               hipStream_t s);
+int kick(mdf_p3m_ctx *c, double *vel, const double *acc, int n, double dt, double *k_sum,
+         hipStream_t s);
+int density_modes(mdf_p3m_ctx *c, const double *pos, int n, const double *q, int nq,
+                  double *out, hipStream_t s);
+// Copy a sorted (n,3) array back to the caller's particle order.
+int unpermute3(mdf_p3m_ctx *c, const double *src_sorted, double *dst, int n, hipStream_t s);
+
+// Dynamic LDS above 64 KiB (gfx950 has 160 KiB per workgroup) must be
+// allowed per kernel before launch.
+template <class F>
+inline int allow_lds(F *kernel, size_t bytes) {
+    if (bytes <= 65536) return MDF_OK;
+    hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) {
+        set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize=%zu): %s", bytes, hipGetErrorString(e));
+        return MDF_EHIP;
+    }
+    return MDF_OK;
+}
+
+// Wave (64-lane) helpers.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace mdf

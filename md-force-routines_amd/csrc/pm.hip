@@ -1,0 +1,344 @@
+// Order-6 charge assignment (scatter) and field back-interpolation (gather).
+//
+// Reference: charge_assgn_6_numba, yukawa_pm_ord6.py:5-95, and
+// field_assgn_6_numba, yukawa_pm_ord6.py:212-312.  Same base index
+// ix = floor(x/h), same offset x/h - (ix + 1/2), same six weight polynomials
+// (:29-48, evaluated in Horner form), same nodes ix..ix+5 wrapped once into
+// the mesh (:52-93), same acc = Z * E_p (:308-310).
+//
+// Work is organised per reference cell (the 2x2x2 sub-cell blocks of the
+// sorted particle array, whose particles are contiguous in four runs):
+// * scatter: the block's particles deposit into a private LDS brick of the
+//   mesh with LDS atomics, then the brick is added to the global mesh with
+//   FP64 atomics (one per non-empty brick node instead of 216 per particle);
+// * gather: the block's brick of the three field meshes is staged in LDS
+//   (when it fits), every particle reads its 216 nodes from there, and the
+//   epilogue writes acc = af * (acc_pp + Z E_p) straight to the caller's
+//   particle order, plus the optional mirrors of the reference's scratch
+//   buffers and an optional fused half kick.
+// Summation order differs from the serial reference, so agreement is to
+// rounding (~1e-16 relative), not bitwise.
+#include "mdf_internal.h"
+
+namespace mdf {
+
+// Weights of yukawa_pm_ord6.py:29-48 in Horner form.
+__device__ __forceinline__ void weights6(double x, double w[6]) {
+    w[0] = (1.0 + x * (-10.0 + x * (40.0 + x * (-80.0 + x * (80.0 + x * -32.0))))) * (1.0 / 3840.0);
+    w[1] = (237.0 + x * (-750.0 + x * (840.0 + x * (-240.0 + x * (-240.0 + x * 160.0))))) * (1.0 / 3840.0);
+    w[2] = (841.0 + x * (-770.0 + x * (-440.0 + x * (560.0 + x * (80.0 + x * -160.0))))) * (1.0 / 1920.0);
+    w[3] = (841.0 + x * (770.0 + x * (-440.0 + x * (-560.0 + x * (80.0 + x * 160.0))))) * (1.0 / 1920.0);
+    w[4] = (237.0 + x * (750.0 + x * (840.0 + x * (240.0 + x * (-240.0 + x * -160.0))))) * (1.0 / 3840.0);
+    w[5] = (1.0 + x * (10.0 + x * (40.0 + x * (80.0 + x * (80.0 + x * 32.0))))) * (1.0 / 3840.0);
+}
+
+__device__ __forceinline__ int base_and_weights(double x, double h, double w[6]) {
+    const int i0 = (int)floor(x / h);
+    const double fr = (x - (i0 + 0.5) * h) / h;
+    weights6(fr, w);
+    return i0;
+}
+
+__device__ __forceinline__ int wrapm(int i, int M) {
+    int r = i % M;
+    return r < 0 ? r + M : r;
+}
+
+// Brick origin (mesh node) of block b along one axis: one node of margin
+// below the lowest base index a particle of the block can have.
+__device__ __forceinline__ int brick_origin(int b, double w, double h) { return (int)floor((2 * b) * w / h) - 1; }
+
+// The block's particles: four runs (hy, hz) of two x-adjacent sub-cells.
+struct BlockRuns {
+    int start[4];
+    int off[5];
+};
+
+__device__ __forceinline__ BlockRuns block_runs(const Geom &g, const int *__restrict__ cell_start, int bx, int by,
+                                                int bz) {
+    BlockRuns R;
+    R.off[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int hy = r & 1, hz = r >> 1;
+        const int c0 = 2 * bx + g.nsub[0] * ((2 * by + hy) + g.nsub[1] * (2 * bz + hz));
+        R.start[r] = cell_start[c0];
+        R.off[r + 1] = R.off[r] + (cell_start[c0 + 2] - R.start[r]);
+    }
+    return R;
+}
+
+__device__ __forceinline__ int run_index(const BlockRuns &R, int k) {
+    int s = R.start[0] + k;
+#pragma unroll
+    for (int r = 1; r < 4; ++r)
+        if (k >= R.off[r]) s = R.start[r] + (k - R.off[r]);
+    return s;
+}
+
+struct AssignArgs {
+    Geom g;
+    Brick br;
+    const double4 *__restrict__ spos;
+    const int *__restrict__ cell_start;
+    double *__restrict__ rho;
+    int *__restrict__ stats;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
+    extern __shared__ double brick[];
+    const Geom &g = P.g;
+    const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
+    const int nb = B0 * B1 * B2;
+    const int bid = blockIdx.x;
+    const int bx = bid % g.nblk[0];
+    const int by = (bid / g.nblk[0]) % g.nblk[1];
+    const int bz = bid / (g.nblk[0] * g.nblk[1]);
+    const int o0 = brick_origin(bx, g.w[0], g.h[0]);
+    const int o1 = brick_origin(by, g.w[1], g.h[1]);
+    const int o2 = brick_origin(bz, g.w[2], g.h[2]);
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) brick[t] = 0.0;
+    const BlockRuns R = block_runs(g, P.cell_start, bx, by, bz);
+    __syncthreads();
+
+    for (int k = threadIdx.x; k < R.off[4]; k += blockDim.x) {
+        const double4 p = P.spos[run_index(R, k)];
+        double wx[6], wy[6], wz[6];
+        const int ix = base_and_weights(p.x, g.h[0], wx);
+        const int iy = base_and_weights(p.y, g.h[1], wy);
+        const int iz = base_and_weights(p.z, g.h[2], wz);
+        const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
+        const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+        if (LDS && inside) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                const double zw = p.w * wz[a];
+#pragma unroll
+                for (int b = 0; b < 6; ++b) {
+                    const double zwy = zw * wy[b];
+                    double *row = brick + ((rz + a) * B1 + (ry + b)) * B0 + rx;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) atomicAdd(row + c, zwy * wx[c]);
+                }
+            }
+        } else {
+            if (LDS) atomicAdd(&P.stats[1], 1);
+            for (int a = 0; a < 6; ++a) {
+                const int gz = wrapm(iz + a, g.M[2]);
+                const double zw = p.w * wz[a];
+                for (int b = 0; b < 6; ++b) {
+                    const int gy = wrapm(iy + b, g.M[1]);
+                    const double zwy = zw * wy[b];
+                    for (int c = 0; c < 6; ++c) {
+                        const int gx = wrapm(ix + c, g.M[0]);
+                        unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], zwy * wx[c]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        const double v = brick[t];
+        if (v != 0.0) {
+            const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
+            const int gx = wrapm(o0 + tx, g.M[0]);
+            const int gy = wrapm(o1 + ty, g.M[1]);
+            const int gz = wrapm(o2 + tz, g.M[2]);
+            unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], v);
+        }
+    }
+}
+
+int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
+    (void)n;
+    MDF_TRY(fill_zero(c->rho, c->nmesh, s));
+    AssignArgs P;
+    P.g = c->g;
+    P.br = c->brick;
+    P.spos = c->spos;
+    P.cell_start = c->cell_start;
+    P.rho = c->rho;
+    P.stats = c->stats;
+    if (c->brick.lds_assign) {
+        const size_t lds = sizeof(double) * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
+        MDF_TRY(allow_lds(k_assign<true>, lds));
+        hipLaunchKernelGGL(k_assign<true>, dim3(c->nblock), dim3(kPMThreads), lds, s, P);
+    } else {
+        hipLaunchKernelGGL(k_assign<false>, dim3(c->nblock), dim3(kPMThreads), 0, s, P);
+    }
+    MDF_LAUNCHED();
+    if (rho_out) MDF_HIP(hipMemcpyAsync(rho_out, c->rho, c->nmesh * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return MDF_OK;
+}
+
+// ------------------------------------------------------------------ gather --
+
+struct GatherArgs {
+    Geom g;
+    Brick br;
+    const double4 *__restrict__ spos;
+    const int *__restrict__ cell_start;
+    const int *__restrict__ perm;
+    const double *__restrict__ emesh;   // interleaved (Ex, Ey, Ez)
+    const double *__restrict__ acc_pp_s;
+    double *__restrict__ part;          // K partials when kicking
+    GatherOut o;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
+    extern __shared__ double ebrick[];
+    __shared__ double red[kPMThreads / 64];
+    const Geom &g = P.g;
+    const int B0 = P.br.B[0], B1 = P.br.B[1], B2 = P.br.B[2];
+    const int bid = blockIdx.x;
+    const int bx = bid % g.nblk[0];
+    const int by = (bid / g.nblk[0]) % g.nblk[1];
+    const int bz = bid / (g.nblk[0] * g.nblk[1]);
+    const int o0 = brick_origin(bx, g.w[0], g.h[0]);
+    const int o1 = brick_origin(by, g.w[1], g.h[1]);
+    const int o2 = brick_origin(bz, g.w[2], g.h[2]);
+    if (LDS) {
+        const int nb = B0 * B1 * B2;
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+            const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
+            const size_t node = ((size_t)wrapm(o2 + tz, g.M[2]) * g.M[1] + wrapm(o1 + ty, g.M[1])) * g.M[0] +
+                                wrapm(o0 + tx, g.M[0]);
+            ebrick[3 * t + 0] = P.emesh[3 * node + 0];
+            ebrick[3 * t + 1] = P.emesh[3 * node + 1];
+            ebrick[3 * t + 2] = P.emesh[3 * node + 2];
+        }
+        __syncthreads();
+    }
+    const BlockRuns R = block_runs(g, P.cell_start, bx, by, bz);
+    double ksum = 0.0;
+    for (int k = threadIdx.x; k < R.off[4]; k += blockDim.x) {
+        const int s = run_index(R, k);
+        const double4 p = P.spos[s];
+        double wx[6], wy[6], wz[6];
+        const int ix = base_and_weights(p.x, g.h[0], wx);
+        const int iy = base_and_weights(p.y, g.h[1], wy);
+        const int iz = base_and_weights(p.z, g.h[2], wz);
+        double ex = 0.0, ey = 0.0, ez = 0.0;
+        const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
+        const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+        if (LDS && inside) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+#pragma unroll
+                for (int b = 0; b < 6; ++b) {
+                    const double wzy = wz[a] * wy[b];
+                    const double *row = ebrick + 3 * (((rz + a) * B1 + (ry + b)) * B0 + rx);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        const double ww = wzy * wx[c];
+                        ex += row[3 * c + 0] * ww;
+                        ey += row[3 * c + 1] * ww;
+                        ez += row[3 * c + 2] * ww;
+                    }
+                }
+            }
+        } else {
+            for (int a = 0; a < 6; ++a) {
+                const int gz = wrapm(iz + a, g.M[2]);
+                for (int b = 0; b < 6; ++b) {
+                    const int gy = wrapm(iy + b, g.M[1]);
+                    const double wzy = wz[a] * wy[b];
+                    const double *row = P.emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        const int gx = wrapm(ix + c, g.M[0]);
+                        const double ww = wzy * wx[c];
+                        ex += row[3 * gx + 0] * ww;
+                        ey += row[3 * gx + 1] * ww;
+                        ez += row[3 * gx + 2] * ww;
+                    }
+                }
+            }
+        }
+        const int i = P.perm[s];
+        const double apx = p.w * ex, apy = p.w * ey, apz = p.w * ez;
+        const GatherOut &o = P.o;
+        if (o.ex) o.ex[i] = ex;
+        if (o.ey) o.ey[i] = ey;
+        if (o.ez) o.ez[i] = ez;
+        if (o.acc_pm) {
+            o.acc_pm[3 * (size_t)i + 0] = apx;
+            o.acc_pm[3 * (size_t)i + 1] = apy;
+            o.acc_pm[3 * (size_t)i + 2] = apz;
+        }
+        double ppx = 0.0, ppy = 0.0, ppz = 0.0;
+        if (o.include_pp || o.acc_pp) {
+            ppx = P.acc_pp_s[3 * (size_t)s + 0];
+            ppy = P.acc_pp_s[3 * (size_t)s + 1];
+            ppz = P.acc_pp_s[3 * (size_t)s + 2];
+            if (o.acc_pp) {
+                o.acc_pp[3 * (size_t)i + 0] = ppx;
+                o.acc_pp[3 * (size_t)i + 1] = ppy;
+                o.acc_pp[3 * (size_t)i + 2] = ppz;
+            }
+        }
+        if (o.acc) {
+            double ax, ay, az;
+            if (o.include_pp) {
+                ax = o.af * (ppx + apx);
+                ay = o.af * (ppy + apy);
+                az = o.af * (ppz + apz);
+            } else {
+                ax = apx;
+                ay = apy;
+                az = apz;
+            }
+            o.acc[3 * (size_t)i + 0] = ax;
+            o.acc[3 * (size_t)i + 1] = ay;
+            o.acc[3 * (size_t)i + 2] = az;
+            if (o.vel) {
+                double *v = o.vel + 3 * (size_t)i;
+                const double vx = v[0] + 0.5 * ax * o.kick_dt;
+                const double vy = v[1] + 0.5 * ay * o.kick_dt;
+                const double vz = v[2] + 0.5 * az * o.kick_dt;
+                v[0] = vx;
+                v[1] = vy;
+                v[2] = vz;
+                ksum += vx * vx + vy * vy + vz * vz;
+            }
+        }
+    }
+    if (P.o.vel) {
+        ksum = wave_sum(ksum);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ksum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kPMThreads / 64; ++w) t += red[w];
+            P.part[bid] = t;
+        }
+    }
+}
+
+int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
+    (void)n;
+    GatherArgs P;
+    P.g = c->g;
+    P.br = c->brick;
+    P.spos = c->spos;
+    P.cell_start = c->cell_start;
+    P.perm = c->perm;
+    P.emesh = c->emesh;
+    P.acc_pp_s = c->acc_pp_s;
+    P.part = c->part_red;
+    P.o = o;
+    if (c->brick.lds_gather) {
+        const size_t lds = 3 * sizeof(double) * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
+        MDF_TRY(allow_lds(k_gather<true>, lds));
+        hipLaunchKernelGGL(k_gather<true>, dim3(c->nblock), dim3(kPMThreads), lds, s, P);
+    } else {
+        hipLaunchKernelGGL(k_gather<false>, dim3(c->nblock), dim3(kPMThreads), 0, s, P);
+    }
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+}  // namespace mdf

@@ -1,0 +1,362 @@
+"""Pins of the CPU oracle (no GPU).
+
+Every oracle function is checked against something other than a retyping of
+its own formula: the reference's compiled C particle_particle (golden
+fixtures), brute force, closed forms, limits and invariants.
+"""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+from oracle import c_oracle
+from oracle import p3m_oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "pp_ref_*.npz")))
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean(np.sum((a - b) ** 2, axis=-1)) / np.mean(np.sum(b ** 2, axis=-1))))
+
+
+# ---------------------------------------------------------------- PP ------
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_pp_matches_reference_c_golden(path):
+    """Numpy oracle and C restatement vs the reference's own compiled C PP."""
+    g = np.load(path)
+    pos, Z, L = g["pos"], g["Z"], float(g["L"])
+    kappa, alpha, rc = float(g["kappa"]), float(g["alpha"]), float(g["rc"])
+    unit = bool(np.all(Z == 1.0))
+    U, acc = O.particle_particle(kappa, alpha, rc, [L, L, L], pos, Z=None if unit else Z)
+    assert rel_rms(acc, g["acc"]) < 1e-13
+    assert np.max(np.abs(acc - g["acc"])) < 1e-11 * np.max(np.abs(g["acc"]))
+    if unit:
+        _, acc_c = c_oracle.pp(pos, [L, L, L], kappa, alpha, rc)
+        assert rel_rms(acc_c, g["acc"]) < 1e-13
+
+
+def test_golden_fixtures_present():
+    assert len(GOLDEN) >= 4
+
+
+def test_pp_cell_list_equals_brute_force():
+    N = 600
+    L = O.box_length(N)
+    pos, _ = O.initial_split(5, 6, N, [L] * 3, 0.01)
+    for kappa, alpha, rc in ((2.0, 0.85, L / 3.1), (0.0, 1.1, L / 3.5), (3.0, 0.5, L / 3.0)):
+        U1, a1 = O.particle_particle(kappa, alpha, rc, [L] * 3, pos)
+        U2, a2 = O.particle_particle_brute(kappa, alpha, rc, [L] * 3, pos)
+        assert abs(U1 - U2) <= 1e-13 * abs(U2)
+        assert rel_rms(a1, a2) < 1e-14
+
+
+def test_pp_newton_third_law():
+    N = 1500
+    L = O.box_length(N)
+    pos, _ = O.initial_split(7, 8, N, [L] * 3, 0.01)
+    _, acc = O.particle_particle(2.0, 0.8, L / 3.2, [L] * 3, pos)
+    assert np.max(np.abs(acc.sum(axis=0))) < 1e-10 * np.abs(acc).max()
+
+
+def test_pp_kappa_zero_is_ewald_real_space():
+    r = np.linspace(0.05, 5.0, 400)
+    alpha = 0.9
+    phi, f = O.pp_pair(r, 0.0, alpha)
+    np.testing.assert_allclose(phi, erfc(alpha * r) / r, rtol=1e-14)
+    fr = erfc(alpha * r) / r ** 2 + 2 * alpha / math.sqrt(math.pi) * np.exp(-(alpha * r) ** 2) / r
+    np.testing.assert_allclose(f, fr, rtol=1e-13)
+
+
+def test_pp_pair_force_is_minus_derivative():
+    # f(r) = -dphi/dr, checked by a 4th-order central difference
+    r = np.linspace(0.3, 6.0, 300)
+    for kappa, alpha in ((2.0, 0.85), (0.5, 0.4), (3.0, 1.2)):
+        h = 1e-4
+        p = lambda x: O.pp_pair(x, kappa, alpha)[0]  # noqa: E731
+        d = (-p(r + 2 * h) + 8 * p(r + h) - 8 * p(r - h) + p(r - 2 * h)) / (12 * h)
+        _, f = O.pp_pair(r, kappa, alpha)
+        np.testing.assert_allclose(f, -d, rtol=1e-8, atol=1e-12)
+
+
+def test_pp_pair_large_alpha_limit_is_bare_yukawa():
+    # alpha -> 0 sends the erfc factors to 2 and erfc-weighted terms to the
+    # bare Yukawa pair e^{-kappa r}/r (the short-range part becomes the whole)
+    r = np.linspace(0.5, 4.0, 50)
+    kappa = 1.3
+    phi, f = O.pp_pair(r, kappa, 1e-6)
+    np.testing.assert_allclose(phi, np.exp(-kappa * r) / r, rtol=1e-5)
+    np.testing.assert_allclose(f, np.exp(-kappa * r) * (1 + kappa * r) / r ** 2, rtol=1e-5)
+
+
+def test_pp_two_particles_closed_form():
+    L = 20.0
+    pos = np.array([[1.0, 2.0, 3.0], [3.5, 2.0, 3.0]])
+    kappa, alpha, rc = 1.0, 0.7, 6.0
+    U, acc = O.particle_particle(kappa, alpha, rc, [L] * 3, pos)
+    phi, f = O.pp_pair(np.array([2.5]), kappa, alpha)
+    assert U == pytest.approx(phi[0], rel=1e-15)
+    np.testing.assert_allclose(acc[0], [-f[0], 0, 0], rtol=1e-15)
+    np.testing.assert_allclose(acc[1], [f[0], 0, 0], rtol=1e-15)
+    # across the periodic boundary
+    pos = np.array([[0.5, 2.0, 3.0], [19.0, 2.0, 3.0]])
+    U, acc = O.particle_particle(kappa, alpha, rc, [L] * 3, pos)
+    phi, f = O.pp_pair(np.array([1.5]), kappa, alpha)
+    assert U == pytest.approx(phi[0], rel=1e-14)
+    np.testing.assert_allclose(acc[0], [f[0], 0, 0], rtol=1e-14)
+
+
+# --------------------------------------------------------- assignment ----
+
+def _cox_de_boor(order, t):
+    """Cardinal B-spline M_order(t) on [0, order] by the Cox-de Boor recursion."""
+    t = np.asarray(t, dtype=np.float64)
+    b = ((t >= np.arange(order)[:, None]) & (t < np.arange(1, order + 1)[:, None])).astype(float)
+    for k in range(2, order + 1):
+        nb = np.zeros((order - k + 1, t.size))
+        for i in range(order - k + 1):
+            nb[i] = ((t - i) * b[i] + (i + k - t) * b[i + 1]) / (k - 1)
+        b = nb
+    return b[0]
+
+
+def test_weights_are_cardinal_bspline_order6():
+    # weight g at offset x equals the cardinal B-spline M6 at (5 - g) + 1/2 + x
+    x = np.linspace(-0.5, 0.5, 101, endpoint=False)
+    w = O.bspline6_weights(x)
+    for g in range(6):
+        np.testing.assert_allclose(w[:, g], _cox_de_boor(6, (5 - g) + 0.5 + x), atol=3e-16)
+    np.testing.assert_allclose(w.sum(axis=1), 1.0, atol=1e-15)
+
+
+def test_charge_assignment_conserves_charge():
+    N = 3000
+    L = O.box_length(N)
+    pos, _ = O.initial_split(3, 4, N, [L] * 3, 0.01)
+    Z = np.random.RandomState(0).choice([1.0, 2.0], N)
+    for M in (16, 32):
+        rho = O.charge_assign(pos, Z, M, M, M, L / M, L / M, L / M)
+        assert rho.sum() == pytest.approx(Z.sum(), rel=1e-14)
+        rho_c = c_oracle.charge_assign(pos, Z, M, L / M)
+        np.testing.assert_allclose(rho, rho_c, atol=1e-14)
+
+
+def test_single_particle_assignment_is_tensor_product():
+    M, L = 16, 10.0
+    h = L / M
+    pos = np.array([[3.3, 7.1, 0.2]])
+    rho = O.charge_assign(pos, np.array([2.0]), M, M, M, h, h, h)
+    ix = np.floor(pos[0] / h).astype(int)
+    fr = (pos[0] - (ix + 0.5) * h) / h
+    w = [O.bspline6_weights(f) for f in fr]
+    exp = np.zeros((M, M, M))
+    for a in range(6):
+        for b in range(6):
+            for c in range(6):
+                exp[(ix[2] + a) % M, (ix[1] + b) % M, (ix[0] + c) % M] += 2.0 * w[2][a] * w[1][b] * w[0][c]
+    np.testing.assert_allclose(rho, exp, atol=1e-16)
+
+
+def test_gather_is_adjoint_of_assignment():
+    N, M = 400, 16
+    L = O.box_length(N)
+    h = L / M
+    rng = np.random.RandomState(9)
+    pos = L * rng.random_sample((N, 3))
+    Z = rng.random_sample(N) + 0.5
+    E = [rng.standard_normal((M, M, M)) for _ in range(3)]
+    acc, Ep = O.field_gather(*E, pos, Z, M, M, M, h, h, h)
+    rho = O.charge_assign(pos, Z, M, M, M, h, h, h)
+    for a in range(3):
+        assert np.sum(Z * Ep[:, a]) == pytest.approx(np.sum(rho * E[a]), rel=1e-12)
+    # constant field: partition of unity
+    acc, Ep = O.field_gather(np.full((M, M, M), 2.5), np.zeros((M, M, M)), np.full((M, M, M), -1.0), pos, Z,
+                             M, M, M, h, h, h)
+    np.testing.assert_allclose(Ep[:, 0], 2.5, rtol=1e-14)
+    np.testing.assert_allclose(Ep[:, 2], -1.0, rtol=1e-14)
+    np.testing.assert_allclose(acc[:, 0], 2.5 * Z, rtol=1e-14)
+
+
+# ----------------------------------------------------------- Green fn ----
+
+def test_green_m0_reduces_to_R_over_U2():
+    M, L, kappa, alpha, p = 8, 9.0, 1.5, 0.8, 6
+    h = L / M
+    G, kx, ky, kz, _ = O.gf_opt(kappa, alpha, p, 0, 0, 0, M, M, M, h, h, h, L, L, L)
+    K2 = kx ** 2 + ky ** 2 + kz ** 2
+    R = 4 * np.pi * np.exp(-(kappa ** 2 + K2) / (4 * alpha ** 2)) / (kappa ** 2 + K2)
+    U = (np.sinc(kx * h / 2 / np.pi) * np.sinc(ky * h / 2 / np.pi) * np.sinc(kz * h / 2 / np.pi)) ** p
+    exp = R / U ** 2
+    exp[K2 == 0] = 0.0
+    np.testing.assert_allclose(G, exp, rtol=1e-13, atol=0)
+
+
+def test_green_symmetric_and_matches_c():
+    M, L = 16, 12.0
+    h = L / M
+    G, *_ , A = O.gf_opt(2.0, 0.85, 6, 3, 3, 3, M, M, M, h, h, h, L, L, L)
+    # k -> -k: shifted index n -> 2*mid - n (for n > 0)
+    Gi = G[1:, 1:, 1:]
+    np.testing.assert_allclose(Gi, Gi[::-1, ::-1, ::-1], rtol=1e-13)
+    Gc, Ac = c_oracle.gf_opt(2.0, 0.85, 6, 3, M, L)
+    np.testing.assert_allclose(G, Gc, rtol=1e-13, atol=1e-300)
+    assert A == pytest.approx(Ac, rel=1e-12)
+
+
+def test_green_fine_mesh_limit():
+    # aliases vanish as h -> 0, leaving G = R / U(k)^2, and U -> 1 as kh -> 0
+    L, kappa, alpha = 10.0, 1.0, 0.6
+    k = 2 * np.pi / L
+    R = 4 * np.pi * np.exp(-(kappa ** 2 + k ** 2) / (4 * alpha ** 2)) / (kappa ** 2 + k ** 2)
+    dev = []
+    for M in (32, 64):
+        h = L / M
+        G, *_ = O.gf_opt(kappa, alpha, 6, 1, 1, 1, M, M, M, h, h, h, L, L, L)
+        mid = M // 2
+        U = np.sinc(k * h / 2 / np.pi) ** 6
+        assert G[mid, mid, mid + 1] == pytest.approx(R / U ** 2, rel=1e-9)
+        dev.append(abs(G[mid, mid, mid + 1] / R - 1))
+    assert dev[1] < dev[0] / 3.9  # O(h^2) approach to R
+
+
+# ----------------------------------------------------------------- FFT / PM --
+
+def test_c_fft_matches_numpy():
+    rng = np.random.RandomState(1)
+    for shape in ((8, 16, 32), (12, 8, 10)):
+        x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        np.testing.assert_allclose(c_oracle.fft3d(x, -1), np.fft.fftn(x), rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(c_oracle.fft3d(x, +1), np.fft.ifftn(x) * x.size, rtol=1e-12, atol=1e-12)
+
+
+def test_pm_plane_wave_known_answer():
+    # rho = cos(2 pi x / L): E_x = (1/V) sum_k -i k G rho_k e^{ikx} = G(k1) (N/V)... analytic
+    M, L = 16, 8.0
+    h = L / M
+    x = (np.arange(M) * h)
+    rho = np.broadcast_to(np.cos(2 * np.pi * x / L)[None, None, :], (M, M, M)).copy()
+    G = np.zeros((M, M, M))
+    mid = M // 2
+    G[mid, mid, mid + 1] = 3.0
+    G[mid, mid, mid - 1] = 3.0
+    kx = 2 * np.pi * (np.arange(M).reshape(1, M) - mid) / L
+    ky = 2 * np.pi * (np.arange(M).reshape(M, 1) - mid) / L
+    kz = 2 * np.pi * (np.arange(M).reshape(M, 1, 1) - mid) / L
+    U, Ex, Ey, Ez = O.pm_solve(rho, G, kx, ky, kz, L, L, L)
+    k1 = 2 * np.pi / L
+    V = L ** 3
+    # rho_k at +-k1 is M^3/2; E_x(x) = (1/V) * 2 * Re(-i k1 * 3 * M^3/2 * e^{i k1 x}) = 3 k1 M^3/V sin(k1 x)
+    exp = 3.0 * k1 * M ** 3 / V * np.sin(k1 * x)
+    np.testing.assert_allclose(Ex[0, 0, :], exp, atol=1e-12)
+    np.testing.assert_allclose(Ey, 0.0, atol=1e-12)
+    # Parseval: U = 1/2 sum |rho_k|^2 G / V = 1/2 * 2 * (M^3/2)^2 * 3 / V
+    assert U == pytest.approx(0.5 * 2 * (M ** 3 / 2) ** 2 * 3.0 / V, rel=1e-12)
+
+
+def test_pm_oracles_agree():
+    N, M = 1500, 32
+    L = O.box_length(N)
+    h = L / M
+    pos, _ = O.initial_split(1, 2, N, [L] * 3, 0.01)
+    Z = np.ones(N)
+    G, kx, ky, kz, _ = O.gf_opt(2.0, 0.85, 6, 3, 3, 3, M, M, M, h, h, h, L, L, L)
+    U1, a1, rho1, E1 = O.particle_mesh_fft_r(2.0, 0.85, pos, Z, N, G, kx, ky, kz, M, M, M, h, h, h, L, L, L)
+    U2, a2, rho2, E2 = c_oracle.pm(pos, Z, G, M, L)
+    assert U1 == pytest.approx(U2, rel=1e-13)
+    assert rel_rms(a1, a2) < 1e-13
+    np.testing.assert_allclose(rho1, rho2, atol=1e-14)
+
+
+def test_pm_converges_to_ewald_reciprocal_sum():
+    # method check: P3M long-range force -> explicit Yukawa-Ewald k-sum
+    N, L, kappa, alpha = 20, 8.0, 1.0, 0.9
+    rng = np.random.RandomState(2)
+    pos = L * rng.random_sample((N, 3))
+    Z = np.ones(N)
+    kmax = 9
+    m = np.arange(-kmax, kmax + 1)
+    KX, KY, KZ = np.meshgrid(m, m, m, indexing="ij")
+    K = 2 * np.pi / L * np.stack([KX.ravel(), KY.ravel(), KZ.ravel()], 1)
+    K = K[np.any(K != 0, axis=1)]
+    K2 = np.sum(K * K, axis=1)
+    R = 4 * np.pi * np.exp(-(kappa ** 2 + K2) / (4 * alpha ** 2)) / (kappa ** 2 + K2)
+    S = np.exp(-1j * pos @ K.T).sum(axis=0)  # sum_j e^{-ik x_j}
+    # E(x_i) = (1/V) sum_k -i k R S(k) e^{ik x_i}
+    ph = np.exp(1j * pos @ K.T)
+    E_ref = np.real((ph * (R * S)[None, :]) @ (-1j * K)) / L ** 3
+    M = 32
+    h = L / M
+    G, kx, ky, kz, _ = O.gf_opt(kappa, alpha, 6, 3, 3, 3, M, M, M, h, h, h, L, L, L)
+    _, acc, _, _ = O.particle_mesh_fft_r(kappa, alpha, pos, Z, N, G, kx, ky, kz, M, M, M, h, h, h, L, L, L)
+    assert rel_rms(acc, E_ref) < 1e-5
+
+
+# ------------------------------------------------------- init / modes / MD --
+
+def test_initial_matches_reference_stream():
+    N, L, T = 50, 5.0, 0.01
+    pos, vel = O.initial(103, N, [L] * 3, T)
+    rng = np.random.RandomState(103)
+    u = [rng.random_sample(N) for _ in range(4)]
+    s = np.sqrt(T / 3)
+    vx = s * np.sqrt(-2 * np.log(u[0])) * np.cos(2 * np.pi * u[1])
+    np.testing.assert_allclose(vel[:, 0], vx - vx.mean(), rtol=0, atol=1e-17)
+    np.testing.assert_allclose(pos[:, 0], L * rng.random_sample(N), rtol=0, atol=0)
+    assert np.abs(vel.mean(axis=0)).max() < 1e-17
+    # NumPy's legacy stream is frozen (NEP 19): first draw of seed 1
+    assert np.random.RandomState(1).random_sample() == 0.417022004702574
+
+
+def test_density_modes_lattice_known_answer():
+    L = 10.0
+    n = 10
+    x = np.arange(n) * L / n
+    pos = np.stack([x, np.zeros(n), np.zeros(n)], 1)
+    dq = 2 * np.pi / L
+    out = O.density_modes(pos, np.array([dq, 10 * dq]))
+    assert abs(out[0, 0]) < 1e-12          # sum of n-th roots of unity
+    assert out[1, 0] == pytest.approx(n)   # q on the lattice's reciprocal vector
+    assert out[0, 1] == pytest.approx(n)   # all y = 0
+
+
+def test_reference_q_list():
+    L = O.box_length(10000)
+    q = O.reference_q_list(L)
+    dq = 2 * np.pi / L
+    assert q.size == 3 * int(2.5 / dq)
+    assert q[0] == dq and q[1] == pytest.approx(np.sqrt(2) * dq) and q[3] == pytest.approx(2 * dq)
+
+
+def test_c_md_steps_match_numpy_oracle():
+    N = 400
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 2.0, 0.9, L / 3.1, 16)
+    pos, vel = O.initial_split(1, 2, N, P.Lv, 0.01)
+    Z = np.ones(N)
+    _, acc, _ = P.force(pos, Z)
+    p1, v1, a1 = pos.copy(), vel.copy(), acc.copy()
+    for _ in range(3):
+        p1, v1, a1, U1 = O.update(p1, v1, a1, 0.1, P, Z)
+    p2, v2, a2 = pos.copy(), vel.copy(), acc.copy()
+    en, _ = c_oracle.md_steps(p2, v2, a2, Z, 3, 0.1, L, 2.0, 0.9, L / 3.1, P.G_k, 16)
+    np.testing.assert_allclose(p2, p1, rtol=0, atol=1e-12)
+    assert rel_rms(a2, a1) < 1e-12
+    assert en[-1, 2] == pytest.approx(U1, rel=1e-12)
+
+
+def test_vscale_branches():
+    N = 200
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 2.0, 0.9, L / 3.1, 16)
+    pos, vel = O.initial_split(1, 2, N, P.Lv, 0.01)
+    _, acc, _ = P.force(pos)
+    _, v, _, _ = O.vscale(pos, vel, acc, 0.02, 5, 0.1, P)
+    assert np.sum(v ** 2) / N == pytest.approx(0.02, rel=1e-12)
+    _, v2, _, _ = O.vscale(pos, vel, acc, 0.02, 2500, 0.1, P)
+    _, vv, _, _ = O.update(pos, vel, acc, 0.1, P)
+    T = np.sum(vv ** 2) / N
+    np.testing.assert_allclose(v2, vv * np.sqrt((20 * 0.02 / T - 1) / 20), rtol=1e-14)
