@@ -30,13 +30,15 @@ struct WrapBox {
 __global__ __launch_bounds__(256) void k_kick_drift(double *__restrict__ pos, double *__restrict__ vel,
                                                     const double *__restrict__ acc, int n3, double dt, int pbc,
                                                     WrapBox wb, const double *__restrict__ scale) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n3) return;
     const int d = i % 3;
+    // separate roundings (no FMA contraction), as the reference's NumPy expressions
     double v = vel[i];
-    if (scale) v = v * (*scale);
-    v = v + 0.5 * acc[i] * dt;
-    double x = pos[i] + v * dt;
+    if (scale) v = (v * *scale);
+    v = v + (0.5 * acc[i]) * dt;
+    double x = (pos[i] + (v * dt));
     if (pbc) {
         if (x > wb.hi[d]) x = x - wb.L[d];
         if (x < wb.lo[d]) x = x + wb.L[d];
@@ -47,10 +49,11 @@ __global__ __launch_bounds__(256) void k_kick_drift(double *__restrict__ pos, do
 
 __global__ __launch_bounds__(256) void k_kick(double *__restrict__ vel, const double *__restrict__ acc, int n3,
                                               double dt, double *__restrict__ part) {
+#pragma clang fp contract(off)
     __shared__ double red[4];
     double s = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += gridDim.x * blockDim.x) {
-        const double v = vel[i] + 0.5 * acc[i] * dt;
+        const double v = vel[i] + (0.5 * acc[i]) * dt;
         vel[i] = v;
         s += v * v;
     }
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(256) void k_rescale(double *__restrict__ vel, int n
         rescale_factor(*k_sum, n, T_desired, it, T_out);
         if (fact_out) *fact_out = f;
     }
-    if (i < n3) vel[i] = vel[i] * f;
+    if (i < n3) vel[i] = (vel[i] * f);
 }
 
 // Density modes: block (b, iq) sums cos and sin of q x_a over its particles.

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
             }
         }
         const int i = P.perm[s];
-        const double apx = p.w * ex, apy = p.w * ey, apz = p.w * ez;
+        const double apx = (p.w * ex), apy = (p.w * ey), apz = (p.w * ez);
         const GatherOut &o = P.o;
         if (o.ex) o.ex[i] = ex;
         if (o.ey) o.ey[i] = ey;
@@ -283,9 +283,9 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
         if (o.acc) {
             double ax, ay, az;
             if (o.include_pp) {
-                ax = o.af * (ppx + apx);
-                ay = o.af * (ppy + apy);
-                az = o.af * (ppz + apz);
+                ax = (o.af * (ppx + apx));
+                ay = (o.af * (ppy + apy));
+                az = (o.af * (ppz + apz));
             } else {
                 ax = apx;
                 ay = apy;
@@ -295,10 +295,11 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
             o.acc[3 * (size_t)i + 1] = ay;
             o.acc[3 * (size_t)i + 2] = az;
             if (o.vel) {
+#pragma clang fp contract(off)
                 double *v = o.vel + 3 * (size_t)i;
-                const double vx = v[0] + 0.5 * ax * o.kick_dt;
-                const double vy = v[1] + 0.5 * ay * o.kick_dt;
-                const double vz = v[2] + 0.5 * az * o.kick_dt;
+                const double vx = v[0] + (0.5 * ax) * o.kick_dt;
+                const double vy = v[1] + (0.5 * ay) * o.kick_dt;
+                const double vz = v[2] + (0.5 * az) * o.kick_dt;
                 v[0] = vx;
                 v[1] = vy;
                 v[2] = vz;
