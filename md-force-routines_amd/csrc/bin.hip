@@ -37,17 +37,14 @@ __global__ void k_bin_count(const double *__restrict__ pos, int n, Geom g, int *
     part_slot[i] = atomicAdd(&cell_count[cell], 1);
 }
 
-// One block of 1024 threads; each thread scans a contiguous run of cells.
-__global__ __launch_bounds__(1024) void k_scan_cells(int *__restrict__ cell_count, int *__restrict__ cell_start,
-                                                     int ncell) {
-    __shared__ int wsum[16];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int per = (ncell + 1023) / 1024;
-    const int lo = min(t * per, ncell), hi = min(lo + per, ncell);
-    int s = 0;
-    for (int i = lo; i < hi; ++i) s += cell_count[i];
-    // inclusive scan inside the wave
-    int v = s;
+// Exclusive scan of the cell counts in three passes over tiles of
+// kScanTile cells: per-tile sums, a scan of the tile sums (one block), and a
+// per-tile scan that adds the tile offset, writes cell_start and resets the
+// counts for the next binning.
+constexpr int kScanTile = 1024;  // 256 threads x 4 cells
+
+__device__ __forceinline__ int block_incl_scan_256(int v, int *wsum) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int u = __shfl_up(v, o, 64);
@@ -55,24 +52,80 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int *__restrict__ cell_coun
     }
     if (lane == 63) wsum[wv] = v;
     __syncthreads();
-    if (t < 64) {
-        int x = t < 16 ? wsum[t] : 0;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const int u = __shfl_up(x, o, 64);
-            if (t >= o) x += u;
-        }
-        if (t < 16) wsum[t] = x;
-    }
+    int add = 0;
+    for (int w = 0; w < wv; ++w) add += wsum[w];
     __syncthreads();
-    int run = v - s + (wv > 0 ? wsum[wv - 1] : 0);
-    for (int i = lo; i < hi; ++i) {
-        const int cnt = cell_count[i];
-        cell_start[i] = run;
-        run += cnt;
-        cell_count[i] = 0;  // ready for the next binning
+    return v + add;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tile_sums(const int *__restrict__ cell_count, int ncell,
+                                                        int *__restrict__ tile_sum) {
+    __shared__ int wsum[4];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s += (base + q < ncell) ? cell_count[base + q] : 0;
+    const int incl = block_incl_scan_256(s, wsum);
+    if (threadIdx.x == 255) tile_sum[blockIdx.x] = incl;
+}
+
+// One block of 1024 threads scans up to 1024 * 8 tile sums in place (exclusive).
+__global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ tile_sum, int ntile,
+                                                     int *__restrict__ total) {
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    constexpr int per = 8;
+    int v[per];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+        const int k = t * per + q;
+        v[q] = k < ntile ? tile_sum[k] : 0;
+        s += v[q];
     }
-    if (t == 1023) cell_start[ncell] = wsum[15];
+    int incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int add = 0;
+    for (int w = 0; w < wv; ++w) add += wsum[w];
+    int run = incl - s + add;
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+        const int k = t * per + q;
+        if (k < ntile) tile_sum[k] = run;
+        run += v[q];
+    }
+    if (t == 1023) *total = run;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(int *__restrict__ cell_count, int ncell,
+                                                    const int *__restrict__ tile_off, const int *__restrict__ total,
+                                                    int *__restrict__ cell_start) {
+    __shared__ int wsum[4];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[q] = (base + q < ncell) ? cell_count[base + q] : 0;
+        s += v[q];
+    }
+    const int incl = block_incl_scan_256(s, wsum);
+    int run = tile_off[blockIdx.x] + incl - s;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < ncell) {
+            cell_start[base + q] = run;
+            cell_count[base + q] = 0;  // ready for the next binning
+        }
+        run += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cell_start[ncell] = *total;
 }
 
 __global__ void k_bin_place(const double *__restrict__ pos, const double *__restrict__ Z, int n, Geom g,
@@ -125,7 +178,14 @@ int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hip
     hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, s, pos, n, c->g, c->cell_count, c->part_cell,
                        c->part_slot);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, c->cell_count, c->cell_start, c->ncell);
+    const int ntile = (c->ncell + kScanTile - 1) / kScanTile;
+    int *tile = c->scan_tmp;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntile), dim3(256), 0, s, c->cell_count, c->ncell, tile);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile, ntile, tile + ntile);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(256), 0, s, c->cell_count, c->ncell, tile, tile + ntile,
+                       c->cell_start);
     MDF_LAUNCHED();
     hipLaunchKernelGGL(k_bin_place, dim3(nb), dim3(256), 0, s, pos, Z, n, c->g, c->cell_start, c->part_cell,
                        c->part_slot, c->spos, c->perm);

@@ -79,7 +79,7 @@ static int dmalloc(T **p, size_t count) {
 static void free_ctx(mdf_p3m_ctx *c) {
     if (!c) return;
     prof_free(c);
-    void *ptrs[] = {c->cell_count, c->cell_start, c->part_cell, c->part_slot, c->perm,    c->spos,
+    void *ptrs[] = {c->cheb, c->scan_tmp, c->cell_count, c->cell_start, c->part_cell, c->part_slot, c->perm,    c->spos,
                     c->acc_pp_s,   c->rho,        c->ck,        c->ek,        c->emesh,   c->green,
                     c->tw[0],      c->tw[1],      c->tw[2],     c->part_pp,   c->part_pm, c->part_red,
                     c->scal,       c->stats};
@@ -173,6 +173,11 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     g.rc = p->rc;
     c->ncell = g.nsub[0] * g.nsub[1] * g.nsub[2];
     c->nblock = g.nblk[0] * g.nblk[1] * g.nblk[2];
+    if (c->ncell > 1024 * 8192) {
+        delete c;
+        set_error("too many cells (%d): increase rc", c->ncell);
+        return MDF_EINVAL;
+    }
     if (c->nblock > kMaxPartials) {
         delete c;
         set_error("too many cells (%d > %d): increase rc", c->nblock, kMaxPartials);
@@ -187,8 +192,20 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     const double tcap = (150.0 * 1024.0 - 5120.0) / 24.0;
     if (tm > tcap) tm = tcap;
     if (tm < 512.0) tm = 512.0;
-    c->tile_max = (int)tm;
-    c->pp_lds = 24 * (size_t)c->tile_max + 4 * (224 + kPPWaves * kPPQueue);
+    c->tile_max = ((int)tm + 3) & ~3;  // keeps the table after the tile 16-byte aligned
+    std::vector<double2> pair_tab;
+    {
+        const int rr = build_pair_table(p->kappa, p->alpha, p->rc, pair_tab, c->npiece);
+        if (rr != MDF_OK) {
+            delete c;
+            return rr;
+        }
+    }
+    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece);
+    if (c->pp_lds > 160 * 1024) {
+        c->tile_max -= (int)((c->pp_lds - 160 * 1024) / 24 + 4) & ~3;
+        c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece);
+    }
 
     // per-block mesh brick (assignment / gather)
     for (int d = 0; d < 3; ++d) c->brick.B[d] = (int)ceil(2.0 * g.w[d] / g.h[d]) + 8;
@@ -202,11 +219,13 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     if (r == MDF_OK) r = dmalloc(&(ptr), (cnt))
     ALLOC(c->cell_count, (size_t)c->ncell);
     ALLOC(c->cell_start, (size_t)c->ncell + 1);
+    ALLOC(c->scan_tmp, (size_t)c->ncell / 1024 + 2);
     ALLOC(c->part_cell, (size_t)p->n_max);
     ALLOC(c->part_slot, (size_t)p->n_max);
     ALLOC(c->perm, (size_t)p->n_max);
     ALLOC(c->spos, (size_t)p->n_max);
     ALLOC(c->acc_pp_s, 3 * (size_t)p->n_max);
+    ALLOC(c->cheb, pair_tab.size());
     ALLOC(c->rho, c->nmesh);
     ALLOC(c->ck, c->nspec);
     ALLOC(c->ek, 3 * c->nspec);
@@ -222,6 +241,12 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     if (r != MDF_OK) {
         free_ctx(c);
         return r;
+    }
+    if (hipMemcpy(c->cheb, pair_tab.data(), pair_tab.size() * sizeof(double2), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        set_error("pair-table upload failed");
+        free_ctx(c);
+        return MDF_EHIP;
     }
     // twiddles exp(-2 pi i t / M), evaluated in long double
     for (int d = 0; d < 3; ++d) {

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/mdforce.h"
 
@@ -61,8 +62,8 @@ struct Brick {
 
 constexpr int kPPThreads = 256;     // 4 waves share one 6x6x6 sub-cell tile
 constexpr int kPPWaves = kPPThreads / 64;
-constexpr int kPPQueue = 128;       // per-wave in-cutoff pair queue
 constexpr int kPMThreads = 256;
+constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces
 constexpr int kMaxPartials = 1 << 20;
 
 // ctx->stats (device ints): [0] PP tiles needing more than one LDS pass,
@@ -86,11 +87,14 @@ struct mdf_p3m_ctx {
     // binning
     int *cell_count;    // ncell, zero between steps
     int *cell_start;    // ncell + 1
+    int *scan_tmp;      // per-tile sums of the count scan (+1 total)
     int *part_cell;     // n_max
     int *part_slot;     // n_max
     int *perm;          // sorted -> caller index
     double4 *spos;      // sorted (x, y, z, Z), positions wrapped into [0, L]
     double *acc_pp_s;   // sorted PP acceleration (n, 3)
+    double2 *cheb;      // PP pair-term tables, npiece * (kChebDeg + 1) (g1, g2) pairs
+    int npiece;
 
     // mesh
     double *rho;        // nmesh
@@ -151,6 +155,10 @@ int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, 
 int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, double *A_pm,
                 hipStream_t s);
 int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
+
+// PP pair-term tables (host) and the PP kernel's dynamic LDS size.
+int build_pair_table(double kappa, double alpha, double rc, std::vector<double2> &tab, int &npiece);
+size_t pp_lds_bytes(int tile_max, int npiece);
 
 // Fill helpers (kernels, so they can live inside captured graphs).
 int fill_zero(double *p, size_t n, hipStream_t s);

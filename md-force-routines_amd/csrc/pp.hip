@@ -3,29 +3,36 @@
 // Reference: particle_particle, yukawa_pp.py:6-167 (C twin yukawa_p3m.c:182-416).
 // Same pair set (every pair with minimum-image distance r < rc, rc <= L/3),
 // same pair terms (yukawa_pp.py:143-152), same displacement convention
-// d = x_i - (x_j + shift) (:133-135).  Differences are only in how the work is
-// organised and in summation order:
+// d = x_i - (x_j + shift) (:133-135), same cutoff test sqrt(d.d) < rc (:137-140).
+// Only the organisation of the work and the summation order differ:
 //
-// * One workgroup (8 waves) per reference cell = a 2x2x2 block of half-width
-//   sub-cells.  The 6x6x6 sub-cells around the block (the block plus two
-//   sub-cells on every side) are staged in LDS with their periodic image
-//   shift already applied, as 36 rows of 6 sub-cells; a row is one or two
-//   contiguous runs of the sorted particle array.
-// * Each wave owns one i at a time and sweeps the 5x5 rows of i's stencil,
-//   culling rows (and the outer sub-cells of a row) that lie farther than rc
-//   from i.  Lanes test 64 candidates at once; in-cutoff j are compacted
-//   into a per-wave LDS queue with ballot + popcount, and the expensive pair
-//   terms run only on full 64-entry queue batches, so lanes stay busy
-//   despite the ~20% in-cutoff rate of a cell stencil.
+// * Pair terms.  With b = kappa / (2 alpha) the reference's energy and force
+//   terms are phi(r) = g1(r) / r and f(r) = g2(r) / r^2 with
+//     g1 = 1/2 [e^{kr} erfc(ar+b) + e^{-kr} erfc(ar-b)],
+//     g2 = 1/2 [e^{kr} erfc(ar+b)(1-kr) + e^{-kr} erfc(ar-b)(1+kr)]
+//          + (2a/sqrt(pi)) r e^{-(ar)^2 - b^2}
+//   (e^{-(ar+b)^2} e^{kr} = e^{-(ar-b)^2} e^{-kr} = e^{-(ar)^2-b^2}).  g1 and g2
+//   are entire functions of r, so on [0, rc] they are represented by
+//   piecewise Chebyshev series (equal pieces, degree kChebDeg) fitted per
+//   context on the host from long-double evaluations and checked there to
+//   <= 4e-15 of max|g| (pair_table.hip).  A pair then costs one sqrt, one
+//   division and two Clenshaw recurrences instead of three exponentials and
+//   two complementary error functions.
+// * Cells.  One workgroup per reference cell = a 2x2x2 block of half-width
+//   sub-cells.  The 6x6x6 sub-cells around the block are staged in LDS with
+//   their periodic image shift applied, as 36 rows of 6 sub-cells (one or two
+//   contiguous runs of the sorted particle array each).
+// * Sweep.  A wave handles two i at a time, one per 32-lane half.  Each half
+//   sweeps the stencil rows of its i (5x5 rows of up to 5 sub-cells, with
+//   rows and outer sub-cells farther than rc culled), testing 32 candidates
+//   per step; in-cutoff j are compacted into the half's LDS queue with a
+//   ballot and popcount, and the pair terms run only on queue batches, so
+//   lanes stay busy despite the ~20% in-cutoff rate of a cell stencil.
 // * Each pair is evaluated from both ends (no Newton-3 scatter, no atomics);
-//   i's force and energy are summed across the wave with shuffles.  The
-//   energy is halved at the end: U_s_r = 1/2 sum_i sum_{j != i} phi.
-// * Pair terms use e^{+kr} erfc(ar + b) = erfcx(ar + b) e^{-(ar)^2 - b^2} and
-//   e^{-kr} erfc(ar - b) = erfcx(ar - b) e^{-(ar)^2 - b^2} (b = kappa/2alpha),
-//   and e^{-(ar+b)^2} e^{kr} = e^{-(ar-b)^2} e^{-kr} = e^{-(ar)^2 - b^2}: one exp
-//   and two erfcx per pair instead of eight exp and four erfc.
-// * If a tile exceeds the LDS capacity the rows are processed in several
-//   passes (i's force accumulates in global memory across passes).
+//   i's force and energy are summed across its half-wave with shuffles.
+//   The energy is halved at the end: U_s_r = 1/2 sum_i sum_{j != i} phi.
+// * A tile larger than the LDS capacity is processed in several passes over
+//   its rows (i's force accumulates in global memory across passes).
 #include "mdf_internal.h"
 
 namespace mdf {
@@ -34,41 +41,53 @@ struct PPArgs {
     Geom g;
     const double4 *__restrict__ spos;
     const int *__restrict__ cell_start;
-    double *__restrict__ acc;     // sorted (n, 3)
-    double *__restrict__ part;    // per-block energy partial
+    const double2 *__restrict__ cheb;  // [piece][k] = (g1 coeff, g2 coeff)
+    double *__restrict__ acc;          // sorted (n, 3)
+    double *__restrict__ part;         // per-block energy partial
     int *__restrict__ stats;
     int tile_max;
-    double b;                     // kappa / (2 alpha)
-    double b2;
-    double two_alpha_sqrtpi;      // 2 alpha / sqrt(pi)
-    double rc2m;                  // candidate test radius^2 (rc slightly widened)
+    int npiece;
+    double piece_w, inv_piece_w;
+    double rc2m;                       // candidate test radius^2 (rc slightly widened)
 };
 
-__device__ __forceinline__ void pp_pair_fp64(const PPArgs &P, double dx, double dy, double dz, double &fx,
-                                             double &fy, double &fz, double &u) {
+constexpr int kGroups = 2;             // i per wave
+constexpr int kGL = 64 / kGroups;      // lanes per i
+constexpr int kQCap = 2 * kGL;         // queue entries per group
+
+// Clenshaw sums of both series of one piece at t in [-1, 1].
+__device__ __forceinline__ void cheb2(const double2 *__restrict__ c, double t, double &g1, double &g2) {
+    const double t2 = t + t;
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = kChebDeg; k >= 1; --k) {
+        const double2 ck = c[k];
+        const double na = fma(t2, a1, ck.x - a2);
+        const double nb = fma(t2, b1, ck.y - b2);
+        a2 = a1;
+        a1 = na;
+        b2 = b1;
+        b1 = nb;
+    }
+    g1 = fma(t, a1, c[0].x - a2);
+    g2 = fma(t, b1, c[0].y - b2);
+}
+
+__device__ __forceinline__ void pp_pair_cheb(const PPArgs &P, const double2 *__restrict__ tab, double dx, double dy,
+                                             double dz, double &fx, double &fy, double &fz, double &u) {
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     if (!(r < P.g.rc)) return;  // the reference's exact test, yukawa_pp.py:140
+    int p = (int)(r * P.inv_piece_w);
+    p = p < P.npiece - 1 ? p : P.npiece - 1;
+    const double t = (r - p * P.piece_w) * (2.0 * P.inv_piece_w) - 1.0;
+    double g1, g2;
+    cheb2(tab + p * (kChebDeg + 1), t, g1, g2);
     const double ir = 1.0 / r;
-    const double ar = P.g.alpha * r;
-    const double ge = exp(-(ar * ar) - P.b2);
-    const double ep = erfcx(ar + P.b) * ge;  // e^{kr} erfc(ar + b)
-    const double em = erfcx(ar - P.b) * ge;  // e^{-kr} erfc(ar - b)
-    const double kr = P.g.kappa * r;
-    u += 0.5 * ir * (ep + em);
-    const double f = 0.5 * ir * ir * (ep * (1.0 - kr) + em * (1.0 + kr)) + P.two_alpha_sqrtpi * ir * ge;
-    const double fr = f * ir;
+    u += g1 * ir;
+    const double fr = g2 * ir * ir * ir;
     fx += fr * dx;
     fy += fr * dy;
     fz += fr * dz;
-}
-
-// Broadcast lane 0's value (the caller guarantees it is wave-uniform) so the
-// compiler keeps it in scalar registers.
-__device__ __forceinline__ double uniform(double v) {
-    const unsigned long long b = __double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(b & 0xffffffffull));
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, double &shift) {
@@ -93,24 +112,29 @@ __device__ __forceinline__ double slab_dist(int d, double o, double w) {
     return r > 0.0 ? r : 0.0;
 }
 
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
 __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
     extern __shared__ double lds[];
     const int T = P.tile_max;
     double *tx = lds;
     double *ty = tx + T;
     double *tz = ty + T;
-    int *cum = (int *)(tz + T);          // 217 prefix offsets of the tile sub-cells
-    int *queue = cum + 224;              // kPPWaves * kPPQueue
+    double2 *tab = (double2 *)(tz + T);                    // npiece * (kChebDeg + 1)
+    int *cum = (int *)(tab + P.npiece * (kChebDeg + 1));   // 217 prefix offsets of the tile sub-cells
+    int *queue = cum + 224;                                // kPPWaves * kGroups * kQCap
     __shared__ int tcell[216];
     __shared__ double red[kPPWaves];
 
     const Geom &g = P.g;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = lane / kGL, gl = lane % kGL;
     const int bid = blockIdx.x;
     const int bx = bid % g.nblk[0];
     const int by = (bid / g.nblk[0]) % g.nblk[1];
     const int bz = bid / (g.nblk[0] * g.nblk[1]);
 
+    for (int t = tid; t < P.npiece * (kChebDeg + 1); t += blockDim.x) tab[t] = P.cheb[t];
     // 1. counts of the 216 tile sub-cells, tile order x fastest
     if (tid < 216) {
         const int txs = tid % 6, tys = (tid / 6) % 6, tzs = tid / 36;
@@ -129,10 +153,10 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
         int v[4];
         int s = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = lane * 4 + q;
-            v[q] = k < 216 ? cum[k + 1] : 0;
-            s += v[q];
+        for (int qq = 0; qq < 4; ++qq) {
+            const int k = lane * 4 + qq;
+            v[qq] = k < 216 ? cum[k + 1] : 0;
+            s += v[qq];
         }
         int incl = s;
 #pragma unroll
@@ -142,10 +166,10 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
         }
         int run = incl - s;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = lane * 4 + q;
+        for (int qq = 0; qq < 4; ++qq) {
+            const int k = lane * 4 + qq;
             if (k < 216) cum[k] = run;
-            run += v[q];
+            run += v[qq];
         }
         if (lane == 63) cum[216] = incl;
     }
@@ -160,7 +184,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
         hoff[h + 1] = hoff[h] + (cum[idx + 1] - cum[idx]);
     }
     const int n_home = hoff[8];
-    int *q = queue + wave * kPPQueue;
+    int *q = queue + (wave * kGroups + grp) * kQCap;
     const double rc2m = P.rc2m;
     double u_acc = 0.0;
 
@@ -171,8 +195,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
         while (r1 < 36 && cum[6 * (r1 + 1)] - base <= T) ++r1;
         if (r1 == r0) {  // a single row larger than the LDS tile: unsupported density
             if (tid == 0) atomicOr(&P.stats[2], 1);
-            r1 = r0 + 1;
-            r0 = r1;
+            r0 = r0 + 1;
             continue;
         }
         if (pass == 1 && tid == 0) atomicAdd(&P.stats[0], 1);
@@ -211,43 +234,57 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
         }
         __syncthreads();
 
-        // 4. one i per wave
-        for (int k = wave; k < n_home; k += kPPWaves) {
+        // 4. two i per wave, one per half
+        for (int kb = wave * kGroups; kb < n_home; kb += kPPWaves * kGroups) {
+            const int k = kb + grp;
+            const bool have_i = k < n_home;
+            const int kk = have_i ? k : kb;  // a valid home particle for the idle half
             int h = 0, hb = 0;
 #pragma unroll
             for (int hh = 1; hh < 8; ++hh)
-                if (k >= hoff[hh]) {
+                if (kk >= hoff[hh]) {
                     h = hh;
                     hb = hoff[hh];
                 }
             const int hx = h & 1, hy = (h >> 1) & 1, hz = h >> 2;
             const int hidx = (2 + hx) + 6 * ((2 + hy) + 6 * (2 + hz));
-            const int gi = __builtin_amdgcn_readfirstlane(P.cell_start[tcell[hidx]] + (k - hb));
+            const int gi = P.cell_start[tcell[hidx]] + (kk - hb);
             const double4 pi = P.spos[gi];
-            const double xi = uniform(pi.x), yi = uniform(pi.y), zi = uniform(pi.z);
+            const double xi = pi.x, yi = pi.y, zi = pi.z;
             const double ox = xi - (2 * bx + hx) * g.w[0];
             const double oy = yi - (2 * by + hy) * g.w[1];
             const double oz = zi - (2 * bz + hz) * g.w[2];
             double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
-            int qn = 0;
-            for (int dz = -2; dz <= 2; ++dz) {
+            int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
+            // rows needed by either half: ty in [hy, hy+4], tz in [hz, hz+4]
+            const int hy0 = readlane_i(hy, 0), hy1 = readlane_i(hy, kGL);
+            const int hz0 = readlane_i(hz, 0), hz1 = readlane_i(hz, kGL);
+            const int tz_lo = min(hz0, hz1), tz_hi = max(hz0, hz1) + 4;
+            const int ty_lo = min(hy0, hy1), ty_hi = max(hy0, hy1) + 4;
+            for (int tzr = tz_lo; tzr <= tz_hi; ++tzr) {
+                const int dz = tzr - 2 - hz;
                 const double sdz = slab_dist(dz, oz, g.w[2]);
                 const double d2z = sdz * sdz;
-                if (d2z > rc2m) continue;
-                for (int dy = -2; dy <= 2; ++dy) {
+                for (int tyr = ty_lo; tyr <= ty_hi; ++tyr) {
+                    const int r = tyr + 6 * tzr;
+                    if (r < r0 || r >= r1) continue;
+                    const int dy = tyr - 2 - hy;
                     const double sdy = slab_dist(dy, oy, g.w[1]);
                     const double d2yz = sdy * sdy + d2z;
-                    if (d2yz > rc2m) continue;
-                    const int r = (2 + hy + dy) + 6 * (2 + hz + dz);
-                    if (r < r0 || r >= r1) continue;
-                    const double sxl = ox + g.w[0];
-                    const double sxr = 2.0 * g.w[0] - ox;
-                    const int xa = hx + ((sxl * sxl + d2yz > rc2m) ? 1 : 0);
-                    const int xb = hx + 4 - ((sxr * sxr + d2yz > rc2m) ? 1 : 0);
-                    const int lo = cum[6 * r + xa] - base;
-                    const int hi = cum[6 * r + xb + 1] - base;
-                    for (int j0 = lo; j0 < hi; j0 += 64) {
-                        const int j = j0 + lane;
+                    const bool need = have_i && dy >= -2 && dy <= 2 && dz >= -2 && dz <= 2 && d2yz <= rc2m;
+                    int lo = 0, hi = 0;
+                    if (need) {
+                        const double sxl = ox + g.w[0];
+                        const double sxr = 2.0 * g.w[0] - ox;
+                        const int xa = hx + ((sxl * sxl + d2yz > rc2m) ? 1 : 0);
+                        const int xb = hx + 4 - ((sxr * sxr + d2yz > rc2m) ? 1 : 0);
+                        lo = cum[6 * r + xa] - base;
+                        hi = cum[6 * r + xb + 1] - base;
+                    }
+                    const int len = hi - lo;
+                    const int nit = (max(readlane_i(len, 0), readlane_i(len, kGL)) + kGL - 1) / kGL;
+                    for (int it = 0; it < nit; ++it) {
+                        const int j = lo + it * kGL + gl;
                         bool hit = false;
                         if (j < hi) {
                             const double ddx = xi - tx[j], ddy = yi - ty[j], ddz = zi - tz[j];
@@ -255,28 +292,38 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
                             hit = (r2 < rc2m) && (r2 > 0.0);
                         }
                         const unsigned long long m = __ballot(hit);
-                        if (hit) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = j;
-                        qn += __popcll(m);
-                        if (qn >= 64) {
-                            const int jj = q[lane];
-                            pp_pair_fp64(P, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
-                            const int rest = qn - 64;
-                            const int moved = lane < rest ? q[64 + lane] : 0;
-                            if (lane < rest) q[lane] = moved;
-                            qn = rest;
+                        const unsigned mh = grp ? (unsigned)(m >> 32) : (unsigned)m;
+                        const int qn = grp ? qn1 : qn0;
+                        if (hit) q[qn + __popc(mh & ((1u << gl) - 1u))] = j;
+                        qn0 += __popc((unsigned)m);
+                        qn1 += __popc((unsigned)(m >> 32));
+                        if (qn0 >= kGL || qn1 >= kGL) {
+                            const int cnt = min(grp ? qn1 : qn0, kGL);
+                            if (gl < cnt) {
+                                const int jj = q[gl];
+                                pp_pair_cheb(P, tab, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
+                            }
+                            const int rest = (grp ? qn1 : qn0) - cnt;
+                            const int moved = gl < rest ? q[kGL + gl] : 0;
+                            if (gl < rest) q[gl] = moved;
+                            qn0 = max(qn0 - kGL, 0);
+                            qn1 = max(qn1 - kGL, 0);
                         }
                     }
                 }
             }
-            if (lane < qn) {
-                const int jj = q[lane];
-                pp_pair_fp64(P, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
+            if (gl < (grp ? qn1 : qn0)) {
+                const int jj = q[gl];
+                pp_pair_cheb(P, tab, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
             }
-            fx = wave_sum(fx);
-            fy = wave_sum(fy);
-            fz = wave_sum(fz);
-            u = wave_sum(u);
-            if (lane == 0) {
+#pragma unroll
+            for (int o = kGL / 2; o > 0; o >>= 1) {
+                fx += __shfl_xor(fx, o, 64);
+                fy += __shfl_xor(fy, o, 64);
+                fz += __shfl_xor(fz, o, 64);
+                u += __shfl_xor(u, o, 64);
+            }
+            if (gl == 0 && have_i) {
                 double *a = P.acc + 3 * (size_t)gi;
                 if (pass == 0) {
                     a[0] = fx;
@@ -294,6 +341,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
         r0 = r1;
         ++pass;
     }
+    u_acc += __shfl_xor(u_acc, kGL, 64);
     if (lane == 0) red[wave] = u_acc;
     __syncthreads();
     if (tid == 0) {
@@ -303,19 +351,25 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
     }
 }
 
+size_t pp_lds_bytes(int tile_max, int npiece) {
+    return 24 * (size_t)tile_max + 16 * (size_t)npiece * (kChebDeg + 1) +
+           4 * (size_t)(224 + kPPWaves * kGroups * kQCap);
+}
+
 int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s) {
     (void)n;
     PPArgs P;
     P.g = c->g;
     P.spos = c->spos;
     P.cell_start = c->cell_start;
+    P.cheb = c->cheb;
     P.acc = c->acc_pp_s;
     P.part = c->part_pp;
     P.stats = c->stats;
     P.tile_max = c->tile_max;
-    P.b = 0.5 * c->g.kappa / c->g.alpha;
-    P.b2 = P.b * P.b;
-    P.two_alpha_sqrtpi = 2.0 * c->g.alpha / sqrt(M_PI);
+    P.npiece = c->npiece;
+    P.piece_w = c->g.rc / c->npiece;
+    P.inv_piece_w = c->npiece / c->g.rc;
     const double rcm = c->g.rc * (1.0 + 1e-12);
     P.rc2m = rcm * rcm;
     MDF_TRY(allow_lds(k_pp_fp64, c->pp_lds));
