@@ -124,11 +124,12 @@ def main():
         del acc_t, d
         torch.cuda.synchronize()
 
-    q = torch.arange(1, a.nq + 1, dtype=torch.float64, device=dev) * (2.0 * math.pi / L)
+    dq = 2.0 * math.pi / L
+    q = torch.arange(1, a.nq + 1, dtype=torch.float64, device=dev) * dq
     Z = None
     # warm-up
     if a.warmup:
-        eng.md_run(pos, vel, acc, a.warmup, a.dt, Z=Z, q=q)
+        eng.md_run(pos, vel, acc, a.warmup, a.dt, Z=Z, q_step=dq, nq=a.nq)
     energies = torch.empty((a.steps, 4), dtype=torch.float64, device=dev)
     nqt = torch.empty((a.steps, a.nq, 3), dtype=torch.complex128, device=dev)
     torch.cuda.synchronize()
@@ -138,7 +139,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
-    eng.md_run(pos, vel, acc, a.steps, a.dt, Z=Z, q=q, energies=energies, nqt=nqt)
+    eng.md_run(pos, vel, acc, a.steps, a.dt, Z=Z, q_step=dq, nq=a.nq, energies=energies, nqt=nqt)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0

@@ -118,10 +118,13 @@ int mdf_density_modes(mdf_p3m_ctx *ctx, const double *pos, int n, const double *
  *   energies (device, nsteps*4): per step (E, K, U, T) with K = kf*sum(v^2),
  *     T = K/(kf*n).
  *   nqt (device complex, nsteps*nq*3) or NULL when nq == 0.
+ *   q_step > 0: the wave numbers are the grid q_m = m*q_step, m = 1..nq (nq <= 64;
+ *     q may be NULL), evaluated by recurrence from one sincos per particle and axis;
+ *     otherwise q holds nq arbitrary wave numbers.
  * pos/vel/acc are updated in place; acc must hold the force of pos on entry. */
 int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const double *Z, int n,
                int nsteps, double dt, double af, double uf, double kf, const double *q, int nq,
-               double *energies, double *nqt, void *stream);
+               double q_step, double *energies, double *nqt, void *stream);
 
 /* Stage timing with HIP events recorded on the call's stream, for
  * measurement only.  mode 1: start (nout = event pool size, 0 = default),

@@ -186,9 +186,10 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     c->nmesh = (size_t)g.M[0] * g.M[1] * g.M[2];
     c->nspec = (size_t)g.M[2] * g.M[1] * g.Mxh;
 
-    // LDS tile capacity of the PP kernel: mean tile + 10 sigma (Poisson) + slack
+    // LDS tile capacity of the PP kernel: mean tile + 6 sigma (Poisson) + slack;
+    // larger tiles take more than one pass
     const double mean_tile = 216.0 * (double)p->n_max / (double)c->ncell;
-    double tm = mean_tile + 10.0 * sqrt(mean_tile) + 128.0;
+    double tm = mean_tile + 6.0 * sqrt(mean_tile) + 64.0;
     const double tcap = (150.0 * 1024.0 - 5120.0) / 24.0;
     if (tm > tcap) tm = tcap;
     if (tm < 512.0) tm = 512.0;
@@ -417,9 +418,9 @@ int mdf_density_modes(mdf_p3m_ctx *ctx, const double *pos, int n, const double *
 }
 
 int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const double *Z, int n, int nsteps, double dt,
-               double af, double uf, double kf, const double *q, int nq, double *energies, double *nqt,
-               void *stream) {
-    if (!ctx || (n > 0 && (!pos || !vel || !acc)) || nsteps < 0 || (nq > 0 && (!q || !nqt))) {
+               double af, double uf, double kf, const double *q, int nq, double q_step, double *energies,
+               double *nqt, void *stream) {
+    if (!ctx || (n > 0 && (!pos || !vel || !acc)) || nsteps < 0 || (nq > 0 && ((!q && !(q_step > 0.0)) || !nqt))) {
         set_error("mdf_md_run: bad argument");
         return MDF_EINVAL;
     }
@@ -430,7 +431,9 @@ int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const do
         MDF_TRY(md_step(ctx, pos, vel, acc, Z, n, dt, af, uf, kf, energies ? energies + 4 * (size_t)it : nullptr, s));
         if (nq > 0) {
             prof_mark(ctx, kStageModes, s);
-            MDF_TRY(density_modes(ctx, pos, n, q, nq, nqt + (size_t)it * nq * 6, s));
+            double *dst = nqt + (size_t)it * nq * 6;
+            if (q_step > 0.0) MDF_TRY(density_modes_grid(ctx, pos, n, q_step, nq, dst, s));
+            else MDF_TRY(density_modes(ctx, pos, n, q, nq, dst, s));
         }
         prof_mark(ctx, kStageEnd, s);
     }

@@ -23,6 +23,17 @@ __device__ __forceinline__ double block_sum_256(double v, double *red4) {
     return r;
 }
 
+// Sum over a block of blockDim.x (<= 1024) threads; red holds 16 doubles.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
 struct WrapBox {
     double L[3], lo[3], hi[3];
 };
@@ -115,6 +126,62 @@ __global__ __launch_bounds__(256) void k_modes(const double *__restrict__ pos, i
     }
 }
 
+// Density modes on a grid q_m = m dq, m = 1..nq (nq <= 64).  Per particle
+// and axis, one sincos gives e^{-i dq x} and the next modes of a chunk of
+// kModeChunk follow by complex multiplication (rounding error grows like m
+// ulps, ~1e-15 at m = 20); the first mode of each later chunk is a fresh
+// sincos.  Threads keep the chunk's sums in registers; each wave reduces
+// them once at the end and writes part[((m-1) * nwaves + wave) * 6 + c],
+// the layout k_modes_final sums.
+constexpr int kModeChunk = 16;
+
+__global__ __launch_bounds__(256) void k_modes_grid(const double *__restrict__ pos, int n, double dq, int nq,
+                                                    double *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int stride = gridDim.x * blockDim.x;
+    for (int m0 = 0; m0 < nq; m0 += kModeChunk) {
+        for (int a = 0; a < 3; ++a) {
+            double ar[kModeChunk], ai[kModeChunk];
+#pragma unroll
+            for (int k = 0; k < kModeChunk; ++k) ar[k] = ai[k] = 0.0;
+            for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+                const double th = dq * pos[3 * j + a];
+                double s1, c1, zs, zc;
+                sincos(-th, &s1, &c1);
+                if (m0 == 0) {
+                    zs = s1;
+                    zc = c1;
+                } else {
+                    sincos(-(double)(m0 + 1) * th, &zs, &zc);
+                }
+                ar[0] += zc;
+                ai[0] += zs;
+#pragma unroll
+                for (int k = 1; k < kModeChunk; ++k) {
+                    const double nc = zc * c1 - zs * s1;
+                    const double ns = zc * s1 + zs * c1;
+                    zc = nc;
+                    zs = ns;
+                    ar[k] += zc;
+                    ai[k] += zs;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kModeChunk; ++k) {
+                const double sr = wave_sum(ar[k]);
+                const double si = wave_sum(ai[k]);
+                const int m = m0 + k;
+                if (lane == 0 && m < nq) {
+                    part[((size_t)m * nw + gw) * 6 + 2 * a] = sr;
+                    part[((size_t)m * nw + gw) * 6 + 2 * a + 1] = si;
+                }
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void k_modes_final(const double *__restrict__ part, int nblk, int nq,
                                                     double *__restrict__ out) {
     const int iq = blockIdx.x;
@@ -141,15 +208,15 @@ struct FinalArgs {
     double *rec;       // 4 or null
 };
 
-__global__ __launch_bounds__(256) void k_finalize(FinalArgs P) {
-    __shared__ double red[4];
+__global__ __launch_bounds__(1024) void k_finalize(FinalArgs P) {
+    __shared__ double red[16];
     double a = 0.0, b = 0.0, k = 0.0;
     for (int i = threadIdx.x; i < P.npp; i += blockDim.x) a += P.pp[i];
     for (int i = threadIdx.x; i < P.npm; i += blockDim.x) b += P.pm[i];
     for (int i = threadIdx.x; i < P.nk; i += blockDim.x) k += P.kp[i];
-    a = block_sum_256(a, red);
-    b = block_sum_256(b, red);
-    k = block_sum_256(k, red);
+    a = block_sum(a, red);
+    b = block_sum(b, red);
+    k = block_sum(k, red);
     if (threadIdx.x == 0) {
         const double Upp = a;
         const double Upm = b * P.half_inv_V;
@@ -194,7 +261,7 @@ int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, 
     P.scal = c->scal;
     P.energies = energies;
     P.rec = nullptr;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, P);
     MDF_LAUNCHED();
     return MDF_OK;
 }
@@ -215,7 +282,7 @@ static int finalize_step(mdf_p3m_ctx *c, int n, double uf, double kf, double *re
     P.scal = c->scal;
     P.energies = nullptr;
     P.rec = rec;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, P);
     MDF_LAUNCHED();
     return MDF_OK;
 }
@@ -262,6 +329,23 @@ int density_modes(mdf_p3m_ctx *c, const double *pos, int n, const double *q, int
     hipLaunchKernelGGL(k_modes, dim3(nb, nq), dim3(256), 0, s, pos, n, q, c->part_red);
     MDF_LAUNCHED();
     hipLaunchKernelGGL(k_modes_final, dim3(nq), dim3(64), 0, s, c->part_red, nb, nq, out);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int nq, double *out, hipStream_t s) {
+    if (nq <= 0) return MDF_OK;
+    if (nq > 64) {
+        set_error("density modes on a q grid: nq = %d > 64", nq);
+        return MDF_EINVAL;
+    }
+    int nb = (n + 255) / 256;
+    if (nb > 256) nb = 256;
+    if (nb < 1) nb = 1;
+    const int nwave = nb * 4;
+    hipLaunchKernelGGL(k_modes_grid, dim3(nb), dim3(256), 0, s, pos, n, dq, nq, c->part_red);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL(k_modes_final, dim3(nq), dim3(64), 0, s, c->part_red, nwave, nq, out);
     MDF_LAUNCHED();
     return MDF_OK;
 }

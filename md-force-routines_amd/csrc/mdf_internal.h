@@ -172,6 +172,8 @@ int kick(mdf_p3m_ctx *c, double *vel, const double *acc, int n, double dt, doubl
          hipStream_t s);
 int density_modes(mdf_p3m_ctx *c, const double *pos, int n, const double *q, int nq,
                   double *out, hipStream_t s);
+int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int nq, double *out,
+                       hipStream_t s);
 // Copy a sorted (n,3) array back to the caller's particle order.
 int unpermute3(mdf_p3m_ctx *c, const double *src_sorted, double *dst, int n, hipStream_t s);
 

@@ -7,7 +7,9 @@
 // the mesh (:52-93), same acc = Z * E_p (:308-310).
 //
 // Work is organised per reference cell (the 2x2x2 sub-cell blocks of the
-// sorted particle array, whose particles are contiguous in four runs):
+// sorted particle array, whose particles are contiguous in four runs).  A
+// cell holds a few tens of particles, so each particle gets a group of eight
+// lanes and six of them take one z-plane (36 nodes) of its 6x6x6 stencil:
 // * scatter: the block's particles deposit into a private LDS brick of the
 //   mesh with LDS atomics, then the brick is added to the global mesh with
 //   FP64 atomics (one per non-empty brick node instead of 216 per particle);
@@ -39,10 +41,17 @@ __device__ __forceinline__ int base_and_weights(double x, double h, double w[6])
     return i0;
 }
 
-__device__ __forceinline__ int wrapm(int i, int M) {
-    int r = i % M;
-    return r < 0 ? r + M : r;
+// Wrap for indices in [-M, 3M) (brick nodes and stencils, since M >= 8 and a
+// brick spans at most M/3 + 9 nodes).
+__device__ __forceinline__ int wrap3(int i, int M) {
+    if (i < 0) i += M;
+    if (i >= M) i -= M;
+    if (i >= M) i -= M;
+    return i;
 }
+
+constexpr int kLanesPerParticle = 8;  // six used: one stencil z-plane each
+constexpr int kParticlesPerPass = kPMThreads / kLanesPerParticle;
 
 // Brick origin (mesh node) of block b along one axis: one node of margin
 // below the lowest base index a particle of the block can have.
@@ -102,7 +111,9 @@ __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
     const BlockRuns R = block_runs(g, P.cell_start, bx, by, bz);
     __syncthreads();
 
-    for (int k = threadIdx.x; k < R.off[4]; k += blockDim.x) {
+    const int sub = threadIdx.x % kLanesPerParticle;  // stencil z-plane of this lane
+    for (int k = threadIdx.x / kLanesPerParticle; k < R.off[4]; k += kParticlesPerPass) {
+        if (sub >= 6) continue;
         const double4 p = P.spos[run_index(R, k)];
         double wx[6], wy[6], wz[6];
         const int ix = base_and_weights(p.x, g.h[0], wx);
@@ -110,30 +121,28 @@ __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
         const int iz = base_and_weights(p.z, g.h[2], wz);
         const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
         const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+        double wza = wz[0];
+#pragma unroll
+        for (int a = 1; a < 6; ++a)
+            if (sub == a) wza = wz[a];
+        const double zw = p.w * wza;
         if (LDS && inside) {
 #pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                const double zw = p.w * wz[a];
+            for (int b = 0; b < 6; ++b) {
+                const double zwy = zw * wy[b];
+                double *row = brick + ((rz + sub) * B1 + (ry + b)) * B0 + rx;
 #pragma unroll
-                for (int b = 0; b < 6; ++b) {
-                    const double zwy = zw * wy[b];
-                    double *row = brick + ((rz + a) * B1 + (ry + b)) * B0 + rx;
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) atomicAdd(row + c, zwy * wx[c]);
-                }
+                for (int c = 0; c < 6; ++c) atomicAdd(row + c, zwy * wx[c]);
             }
         } else {
-            if (LDS) atomicAdd(&P.stats[1], 1);
-            for (int a = 0; a < 6; ++a) {
-                const int gz = wrapm(iz + a, g.M[2]);
-                const double zw = p.w * wz[a];
-                for (int b = 0; b < 6; ++b) {
-                    const int gy = wrapm(iy + b, g.M[1]);
-                    const double zwy = zw * wy[b];
-                    for (int c = 0; c < 6; ++c) {
-                        const int gx = wrapm(ix + c, g.M[0]);
-                        unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], zwy * wx[c]);
-                    }
+            if (LDS && sub == 0) atomicAdd(&P.stats[1], 1);
+            const int gz = wrap3(iz + sub, g.M[2]);
+            for (int b = 0; b < 6; ++b) {
+                const int gy = wrap3(iy + b, g.M[1]);
+                const double zwy = zw * wy[b];
+                for (int c = 0; c < 6; ++c) {
+                    const int gx = wrap3(ix + c, g.M[0]);
+                    unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], zwy * wx[c]);
                 }
             }
         }
@@ -143,9 +152,9 @@ __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
         const double v = brick[t];
         if (v != 0.0) {
             const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
-            const int gx = wrapm(o0 + tx, g.M[0]);
-            const int gy = wrapm(o1 + ty, g.M[1]);
-            const int gz = wrapm(o2 + tz, g.M[2]);
+            const int gx = wrap3(o0 + tx, g.M[0]);
+            const int gy = wrap3(o1 + ty, g.M[1]);
+            const int gz = wrap3(o2 + tz, g.M[2]);
             unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], v);
         }
     }
@@ -204,8 +213,8 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
         const int nb = B0 * B1 * B2;
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
             const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
-            const size_t node = ((size_t)wrapm(o2 + tz, g.M[2]) * g.M[1] + wrapm(o1 + ty, g.M[1])) * g.M[0] +
-                                wrapm(o0 + tx, g.M[0]);
+            const size_t node = ((size_t)wrap3(o2 + tz, g.M[2]) * g.M[1] + wrap3(o1 + ty, g.M[1])) * g.M[0] +
+                                wrap3(o0 + tx, g.M[0]);
             ebrick[3 * t + 0] = P.emesh[3 * node + 0];
             ebrick[3 * t + 1] = P.emesh[3 * node + 1];
             ebrick[3 * t + 2] = P.emesh[3 * node + 2];
@@ -214,52 +223,68 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
     }
     const BlockRuns R = block_runs(g, P.cell_start, bx, by, bz);
     double ksum = 0.0;
-    for (int k = threadIdx.x; k < R.off[4]; k += blockDim.x) {
-        const int s = run_index(R, k);
-        const double4 p = P.spos[s];
-        double wx[6], wy[6], wz[6];
-        const int ix = base_and_weights(p.x, g.h[0], wx);
-        const int iy = base_and_weights(p.y, g.h[1], wy);
-        const int iz = base_and_weights(p.z, g.h[2], wz);
+    const int sub = threadIdx.x % kLanesPerParticle;  // stencil z-plane of this lane
+    const int npass = (R.off[4] + kParticlesPerPass - 1) / kParticlesPerPass;
+    for (int pass = 0; pass < npass; ++pass) {
+        const int k = pass * kParticlesPerPass + threadIdx.x / kLanesPerParticle;
+        const bool live = k < R.off[4];
+        const int s = live ? run_index(R, k) : 0;
         double ex = 0.0, ey = 0.0, ez = 0.0;
-        const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
-        const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
-        if (LDS && inside) {
+        double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (live) {
+            p = P.spos[s];
+            if (sub < 6) {
+                double wx[6], wy[6], wz[6];
+                const int ix = base_and_weights(p.x, g.h[0], wx);
+                const int iy = base_and_weights(p.y, g.h[1], wy);
+                const int iz = base_and_weights(p.z, g.h[2], wz);
+                double wza = wz[0];
 #pragma unroll
-            for (int a = 0; a < 6; ++a) {
+                for (int a = 1; a < 6; ++a)
+                    if (sub == a) wza = wz[a];
+                const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
+                const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+                if (LDS && inside) {
 #pragma unroll
-                for (int b = 0; b < 6; ++b) {
-                    const double wzy = wz[a] * wy[b];
-                    const double *row = ebrick + 3 * (((rz + a) * B1 + (ry + b)) * B0 + rx);
+                    for (int b = 0; b < 6; ++b) {
+                        const double wzy = wza * wy[b];
+                        const double *row = ebrick + 3 * (((rz + sub) * B1 + (ry + b)) * B0 + rx);
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        const double ww = wzy * wx[c];
-                        ex += row[3 * c + 0] * ww;
-                        ey += row[3 * c + 1] * ww;
-                        ez += row[3 * c + 2] * ww;
+                        for (int c = 0; c < 6; ++c) {
+                            const double ww = wzy * wx[c];
+                            ex += row[3 * c + 0] * ww;
+                            ey += row[3 * c + 1] * ww;
+                            ez += row[3 * c + 2] * ww;
+                        }
                     }
-                }
-            }
-        } else {
-            for (int a = 0; a < 6; ++a) {
-                const int gz = wrapm(iz + a, g.M[2]);
-                for (int b = 0; b < 6; ++b) {
-                    const int gy = wrapm(iy + b, g.M[1]);
-                    const double wzy = wz[a] * wy[b];
-                    const double *row = P.emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
+                } else {
+                    const int gz = wrap3(iz + sub, g.M[2]);
+                    for (int b = 0; b < 6; ++b) {
+                        const int gy = wrap3(iy + b, g.M[1]);
+                        const double wzy = wza * wy[b];
+                        const double *row = P.emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        const int gx = wrapm(ix + c, g.M[0]);
-                        const double ww = wzy * wx[c];
-                        ex += row[3 * gx + 0] * ww;
-                        ey += row[3 * gx + 1] * ww;
-                        ez += row[3 * gx + 2] * ww;
+                        for (int c = 0; c < 6; ++c) {
+                            const int gx = wrap3(ix + c, g.M[0]);
+                            const double ww = wzy * wx[c];
+                            ex += row[3 * gx + 0] * ww;
+                            ey += row[3 * gx + 1] * ww;
+                            ez += row[3 * gx + 2] * ww;
+                        }
                     }
                 }
             }
         }
+        // sum the six planes of each particle (groups of eight lanes)
+#pragma unroll
+        for (int o = kLanesPerParticle / 2; o > 0; o >>= 1) {
+            ex += __shfl_xor(ex, o, 64);
+            ey += __shfl_xor(ey, o, 64);
+            ez += __shfl_xor(ez, o, 64);
+        }
+        if (!live || sub != 0) continue;
         const int i = P.perm[s];
-        const double apx = (p.w * ex), apy = (p.w * ey), apz = (p.w * ez);
+        const double apx = p.w * ex, apy = p.w * ey, apz = p.w * ez;
         const GatherOut &o = P.o;
         if (o.ex) o.ex[i] = ex;
         if (o.ey) o.ey[i] = ey;

@@ -3,7 +3,8 @@
 // Reference: particle_particle, yukawa_pp.py:6-167 (C twin yukawa_p3m.c:182-416).
 // Same pair set (every pair with minimum-image distance r < rc, rc <= L/3),
 // same pair terms (yukawa_pp.py:143-152), same displacement convention
-// d = x_i - (x_j + shift) (:133-135), same cutoff test sqrt(d.d) < rc (:137-140).
+// d = x_i - (x_j + shift) (:133-135), same cutoff r < rc (:137-140, tested as
+// d.d < rc^2; the two differ only for r within an ulp of rc).
 // Only the organisation of the work and the summation order differ:
 //
 // * Pair terms.  With b = kappa / (2 alpha) the reference's energy and force
@@ -15,19 +16,20 @@
 //   are entire functions of r, so on [0, rc] they are represented by
 //   piecewise Chebyshev series (equal pieces, degree kChebDeg) fitted per
 //   context on the host from long-double evaluations and checked there to
-//   <= 4e-15 of max|g| (pair_table.hip).  A pair then costs one sqrt, one
-//   division and two Clenshaw recurrences instead of three exponentials and
-//   two complementary error functions.
+//   <= 4e-15 of max|g| (pair_table.hip).  A pair then costs one reciprocal
+//   square root and two Clenshaw recurrences instead of a square root, a
+//   division, three exponentials and two complementary error functions.
 // * Cells.  One workgroup per reference cell = a 2x2x2 block of half-width
 //   sub-cells.  The 6x6x6 sub-cells around the block are staged in LDS with
 //   their periodic image shift applied, as 36 rows of 6 sub-cells (one or two
 //   contiguous runs of the sorted particle array each).
 // * Sweep.  A wave handles two i at a time, one per 32-lane half.  Each half
-//   sweeps the stencil rows of its i (5x5 rows of up to 5 sub-cells, with
-//   rows and outer sub-cells farther than rc culled), testing 32 candidates
-//   per step; in-cutoff j are compacted into the half's LDS queue with a
-//   ballot and popcount, and the pair terms run only on queue batches, so
-//   lanes stay busy despite the ~20% in-cutoff rate of a cell stencil.
+//   sweeps the 5x5 stencil rows (5 sub-cells each, one contiguous LDS range
+//   per row, tabulated once per home sub-cell) of its i, testing 32
+//   candidates per step; in-cutoff j are compacted into the half's LDS queue
+//   with a ballot and popcount, and the pair terms run only on queue
+//   batches, so lanes stay busy despite the ~25% in-cutoff rate of a cell
+//   stencil.
 // * Each pair is evaluated from both ends (no Newton-3 scatter, no atomics);
 //   i's force and energy are summed across its half-wave with shuffles.
 //   The energy is halved at the end: U_s_r = 1/2 sum_i sum_{j != i} phi.
@@ -48,6 +50,7 @@ struct PPArgs {
     int tile_max;
     int npiece;
     double piece_w, inv_piece_w;
+    double rc2;                        // rc^2, the pair cutoff
     double rc2m;                       // candidate test radius^2 (rc slightly widened)
 };
 
@@ -75,14 +78,15 @@ __device__ __forceinline__ void cheb2(const double2 *__restrict__ c, double t, d
 
 __device__ __forceinline__ void pp_pair_cheb(const PPArgs &P, const double2 *__restrict__ tab, double dx, double dy,
                                              double dz, double &fx, double &fy, double &fz, double &u) {
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(r < P.g.rc)) return;  // the reference's exact test, yukawa_pp.py:140
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < P.rc2)) return;  // r < rc (yukawa_pp.py:140) tested as r^2 < rc^2
+    const double ir = rsqrt(r2);
+    const double r = r2 * ir;
     int p = (int)(r * P.inv_piece_w);
     p = p < P.npiece - 1 ? p : P.npiece - 1;
     const double t = (r - p * P.piece_w) * (2.0 * P.inv_piece_w) - 1.0;
     double g1, g2;
     cheb2(tab + p * (kChebDeg + 1), t, g1, g2);
-    const double ir = 1.0 / r;
     u += g1 * ir;
     const double fr = g2 * ir * ir * ir;
     fx += fr * dx;
@@ -103,15 +107,6 @@ __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, doubl
     }
 }
 
-// Distance from offset o (in [0, w)) of i inside its sub-cell to the slab of
-// the sub-cell d steps away along one axis.
-__device__ __forceinline__ double slab_dist(int d, double o, double w) {
-    double r = 0.0;
-    if (d > 0) r = d * w - o;
-    else if (d < 0) r = o - (d + 1) * w;
-    return r > 0.0 ? r : 0.0;
-}
-
 __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
@@ -124,6 +119,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
     int *cum = (int *)(tab + P.npiece * (kChebDeg + 1));   // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // kPPWaves * kGroups * kQCap
     __shared__ int tcell[216];
+    __shared__ int2 rng[8][25];  // per home sub-cell: LDS range of each of its 25 stencil rows
     __shared__ double red[kPPWaves];
 
     const Geom &g = P.g;
@@ -232,6 +228,16 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
                 txs = txe;
             }
         }
+        // row ranges of this pass: home sub-cell h, stencil row (dy, dz) covers
+        // tile x sub-cells hx .. hx+4 of tile row (2+hy+dy, 2+hz+dz)
+        if (tid < 200) {
+            const int h = tid / 25, rr = tid % 25;
+            const int hx = h & 1, hy = (h >> 1) & 1, hz = h >> 2;
+            const int r = (hy + rr % 5) + 6 * (hz + rr / 5);
+            int2 v = make_int2(0, 0);
+            if (r >= r0 && r < r1) v = make_int2(cum[6 * r + hx] - base, cum[6 * r + hx + 5] - base);
+            rng[h][rr] = v;
+        }
         __syncthreads();
 
         // 4. two i per wave, one per half
@@ -251,64 +257,39 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
             const int gi = P.cell_start[tcell[hidx]] + (kk - hb);
             const double4 pi = P.spos[gi];
             const double xi = pi.x, yi = pi.y, zi = pi.z;
-            const double ox = xi - (2 * bx + hx) * g.w[0];
-            const double oy = yi - (2 * by + hy) * g.w[1];
-            const double oz = zi - (2 * bz + hz) * g.w[2];
             double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
-            // rows needed by either half: ty in [hy, hy+4], tz in [hz, hz+4]
-            const int hy0 = readlane_i(hy, 0), hy1 = readlane_i(hy, kGL);
-            const int hz0 = readlane_i(hz, 0), hz1 = readlane_i(hz, kGL);
-            const int tz_lo = min(hz0, hz1), tz_hi = max(hz0, hz1) + 4;
-            const int ty_lo = min(hy0, hy1), ty_hi = max(hy0, hy1) + 4;
-            for (int tzr = tz_lo; tzr <= tz_hi; ++tzr) {
-                const int dz = tzr - 2 - hz;
-                const double sdz = slab_dist(dz, oz, g.w[2]);
-                const double d2z = sdz * sdz;
-                for (int tyr = ty_lo; tyr <= ty_hi; ++tyr) {
-                    const int r = tyr + 6 * tzr;
-                    if (r < r0 || r >= r1) continue;
-                    const int dy = tyr - 2 - hy;
-                    const double sdy = slab_dist(dy, oy, g.w[1]);
-                    const double d2yz = sdy * sdy + d2z;
-                    const bool need = have_i && dy >= -2 && dy <= 2 && dz >= -2 && dz <= 2 && d2yz <= rc2m;
-                    int lo = 0, hi = 0;
-                    if (need) {
-                        const double sxl = ox + g.w[0];
-                        const double sxr = 2.0 * g.w[0] - ox;
-                        const int xa = hx + ((sxl * sxl + d2yz > rc2m) ? 1 : 0);
-                        const int xb = hx + 4 - ((sxr * sxr + d2yz > rc2m) ? 1 : 0);
-                        lo = cum[6 * r + xa] - base;
-                        hi = cum[6 * r + xb + 1] - base;
+            for (int rr = 0; rr < 25; ++rr) {
+                int2 lr = rng[h][rr];
+                if (!have_i) lr = make_int2(0, 0);
+                const int lo = lr.x, hi = lr.y;
+                const int len = hi - lo;
+                const int nit = (max(readlane_i(len, 0), readlane_i(len, kGL)) + kGL - 1) / kGL;
+                for (int it = 0; it < nit; ++it) {
+                    const int j = lo + it * kGL + gl;
+                    bool hit = false;
+                    if (j < hi) {
+                        const double ddx = xi - tx[j], ddy = yi - ty[j], ddz = zi - tz[j];
+                        const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        hit = (r2 < rc2m) && (r2 > 0.0);
                     }
-                    const int len = hi - lo;
-                    const int nit = (max(readlane_i(len, 0), readlane_i(len, kGL)) + kGL - 1) / kGL;
-                    for (int it = 0; it < nit; ++it) {
-                        const int j = lo + it * kGL + gl;
-                        bool hit = false;
-                        if (j < hi) {
-                            const double ddx = xi - tx[j], ddy = yi - ty[j], ddz = zi - tz[j];
-                            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                            hit = (r2 < rc2m) && (r2 > 0.0);
+                    const unsigned long long m = __ballot(hit);
+                    const unsigned mh = grp ? (unsigned)(m >> 32) : (unsigned)m;
+                    const int qn = grp ? qn1 : qn0;
+                    if (hit) q[qn + __popc(mh & ((1u << gl) - 1u))] = j;
+                    qn0 += __popc((unsigned)m);
+                    qn1 += __popc((unsigned)(m >> 32));
+                    if (qn0 >= kGL || qn1 >= kGL) {
+                        const int cnt = min(grp ? qn1 : qn0, kGL);
+                        if (gl < cnt) {
+                            const int jj = q[gl];
+                            pp_pair_cheb(P, tab, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
                         }
-                        const unsigned long long m = __ballot(hit);
-                        const unsigned mh = grp ? (unsigned)(m >> 32) : (unsigned)m;
-                        const int qn = grp ? qn1 : qn0;
-                        if (hit) q[qn + __popc(mh & ((1u << gl) - 1u))] = j;
-                        qn0 += __popc((unsigned)m);
-                        qn1 += __popc((unsigned)(m >> 32));
-                        if (qn0 >= kGL || qn1 >= kGL) {
-                            const int cnt = min(grp ? qn1 : qn0, kGL);
-                            if (gl < cnt) {
-                                const int jj = q[gl];
-                                pp_pair_cheb(P, tab, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
-                            }
-                            const int rest = (grp ? qn1 : qn0) - cnt;
-                            const int moved = gl < rest ? q[kGL + gl] : 0;
-                            if (gl < rest) q[gl] = moved;
-                            qn0 = max(qn0 - kGL, 0);
-                            qn1 = max(qn1 - kGL, 0);
-                        }
+                        const int rest = (grp ? qn1 : qn0) - cnt;
+                        const int moved = gl < rest ? q[kGL + gl] : 0;
+                        if (gl < rest) q[gl] = moved;
+                        qn0 = max(qn0 - kGL, 0);
+                        qn1 = max(qn1 - kGL, 0);
                     }
                 }
             }
@@ -371,6 +352,7 @@ int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s) {
     P.piece_w = c->g.rc / c->npiece;
     P.inv_piece_w = c->npiece / c->g.rc;
     const double rcm = c->g.rc * (1.0 + 1e-12);
+    P.rc2 = c->g.rc * c->g.rc;
     P.rc2m = rcm * rcm;
     MDF_TRY(allow_lds(k_pp_fp64, c->pp_lds));
     hipLaunchKernelGGL(k_pp_fp64, dim3(c->nblock), dim3(kPPThreads), c->pp_lds, s, P);

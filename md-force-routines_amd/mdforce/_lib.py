@@ -50,7 +50,7 @@ SIGNATURES = {
     "mdf_verlet_kick": [_vp, _vp, _vp, _i, _d, _vp, _vp],
     "mdf_thermostat_scale": [_vp, _vp, _i, _vp, _d, _i, _vp, _vp],
     "mdf_density_modes": [_vp, _vp, _i, _vp, _i, _vp, _vp],
-    "mdf_md_run": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _i, _vp, _vp, _vp],
+    "mdf_md_run": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _i, _d, _vp, _vp, _vp],
     "mdf_p3m_stats": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_profile": [_vp, _i, _vp, _i],
 }

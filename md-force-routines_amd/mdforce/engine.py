@@ -218,24 +218,33 @@ class P3M:
               "mdf_density_modes")
         return out
 
-    def md_run(self, pos, vel, acc, nsteps, dt, Z=None, af=1.0 / 3.0, uf=1.0, kf=1.5, q=None,
-               energies=None, nqt=None):
-        """nsteps fused NVE steps in place; returns (energies (nsteps,4): E, K, U, T; nqt or None)."""
+    def md_run(self, pos, vel, acc, nsteps, dt, Z=None, af=1.0 / 3.0, uf=1.0, kf=1.5, q=None, q_step=None,
+               nq=None, energies=None, nqt=None):
+        """nsteps fused NVE steps in place; returns (energies (nsteps,4): E, K, U, T; nqt or None).
+
+        n(q, t) is recorded for the wave numbers ``q`` (tensor), or for the
+        grid q_m = m * q_step, m = 1..nq, when ``q_step`` is given.
+        """
         n = pos.shape[0]
         for t, name in ((pos, "pos"), (vel, "vel"), (acc, "acc")):
             _need(t, name, (n, 3), self.device)
         if energies is None:
             energies = torch.empty((nsteps, 4), dtype=torch.float64, device=self.device)
-        nq = 0
-        if q is not None:
+        nqv = 0
+        step = 0.0
+        if q_step is not None:
+            step = float(q_step)
+            nqv = int(nq)
+            q = None
+        elif q is not None:
             q = q.to(self.device, torch.float64).contiguous()
-            nq = q.shape[0]
-            if nqt is None:
-                nqt = torch.empty((nsteps, nq, 3), dtype=torch.complex128, device=self.device)
+            nqv = q.shape[0]
+        if nqv and nqt is None:
+            nqt = torch.empty((nsteps, nqv, 3), dtype=torch.complex128, device=self.device)
         check(self._lib.mdf_md_run(self._ctx, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(Z), n, int(nsteps), float(dt),
-                                   float(af), float(uf), float(kf), _ptr(q), nq, _ptr(energies),
-                                   _ptr(nqt) if nq else None, self._stream()), "mdf_md_run")
-        return energies, (nqt if nq else None)
+                                   float(af), float(uf), float(kf), _ptr(q), nqv, step, _ptr(energies),
+                                   _ptr(nqt) if nqv else None, self._stream()), "mdf_md_run")
+        return energies, (nqt if nqv else None)
 
 
 def self_energy(n, alpha):

@@ -248,6 +248,11 @@ def test_md_run_short_horizon_vs_c_oracle(P3M):
     pd, vd, ad = dev(pos), dev(vel), dev(acc)
     en, nqt = e.md_run(pd, vd, ad, steps, 0.1, Z=dev(Z), q=dev(q))
     torch.cuda.synchronize()
+    # the same run with n(q, t) on the q grid by recurrence
+    pg, vg, ag = dev(pos), dev(vel), dev(acc)
+    en_g, nqt_g = e.md_run(pg, vg, ag, steps, 0.1, Z=dev(Z), q_step=2 * np.pi / L, nq=20)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(nqt_g.cpu().numpy(), nqt_c, rtol=1e-9, atol=1e-9)
     en = en.cpu().numpy()
     assert np.max(np.abs(pd.cpu().numpy() - pc)) < 1e-11
     assert rel_rms(ad.cpu().numpy(), ac) < 1e-11
