@@ -208,12 +208,49 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece);
     }
 
-    // per-block mesh brick (assignment / gather)
-    for (int d = 0; d < 3; ++d) c->brick.B[d] = (int)ceil(2.0 * g.w[d] / g.h[d]) + 8;
-    const size_t bnodes = (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
-    // bricks too large for LDS fall back to global-memory atomics / reads
-    c->brick.lds_assign = (bnodes * 8 <= 96 * 1024) ? 1 : 0;
-    c->brick.lds_gather = (bnodes * 24 <= 72 * 1024) ? 1 : 0;
+    // PM blocks: the largest cb (<= 4 reference cells per axis) whose brick
+    // fits the LDS budget; a brick spans ceil(cb * cell / h) + 7 nodes (one node
+    // of margin below, the base-index range of the block, the 6-node stencil)
+    {
+        auto extent = [&](int cb, int *B) {
+            size_t nn = 1;
+            for (int d = 0; d < 3; ++d) {
+                const int cbd = cb < g.nblk[d] ? cb : g.nblk[d];
+                B[d] = (int)ceil(cbd * 2.0 * g.w[d] / g.h[d]) + 7;
+                // odd x and y extents: the z-plane stride of the brick is then an
+                // odd multiple of the element size, so the six lanes that work on
+                // the six z-planes of one particle hit different LDS banks
+                if (d < 2) B[d] |= 1;
+                nn *= (size_t)B[d];
+            }
+            return nn;
+        };
+        auto nblocks = [&](int cb) {
+            return ((g.nblk[0] + cb - 1) / cb) * ((g.nblk[1] + cb - 1) / cb) * ((g.nblk[2] + cb - 1) / cb);
+        };
+        c->brick.cb_assign = 1;
+        c->brick.cb_gather = 1;
+        for (int cb = 4; cb >= 1; --cb) {
+            int B[3];
+            if (extent(cb, B) * 8 <= 64 * 1024 || cb == 1) {
+                c->brick.cb_assign = cb;
+                for (int d = 0; d < 3; ++d) c->brick.B[d] = B[d];
+                c->brick.lds_assign = extent(cb, B) * 8 <= 128 * 1024 ? 1 : 0;
+                break;
+            }
+        }
+        for (int cb = 4; cb >= 1; --cb) {
+            int B[3];
+            if (extent(cb, B) * 24 <= 144 * 1024 || cb == 1) {
+                c->brick.cb_gather = cb;
+                for (int d = 0; d < 3; ++d) c->brick.Bg[d] = B[d];
+                c->brick.lds_gather = extent(cb, B) * 24 <= 144 * 1024 ? 1 : 0;
+                break;
+            }
+        }
+        c->brick.nblock_assign = nblocks(c->brick.cb_assign);
+        c->brick.nblock_gather = nblocks(c->brick.cb_gather);
+    }
 
     int r = MDF_OK;
 #define ALLOC(ptr, cnt) \

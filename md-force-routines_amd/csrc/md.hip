@@ -137,48 +137,60 @@ constexpr int kModeChunk = 16;
 
 __global__ __launch_bounds__(256) void k_modes_grid(const double *__restrict__ pos, int n, double dq, int nq,
                                                     double *__restrict__ part) {
+    // blockIdx.y: axis, blockIdx.z: chunk of kModeChunk modes starting at m0
     const int lane = threadIdx.x & 63;
     const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
     const int stride = gridDim.x * blockDim.x;
-    for (int m0 = 0; m0 < nq; m0 += kModeChunk) {
-        for (int a = 0; a < 3; ++a) {
-            double ar[kModeChunk], ai[kModeChunk];
+    const int a = blockIdx.y;
+    const int m0 = blockIdx.z * kModeChunk;
+    double ar[kModeChunk], ai[kModeChunk];
 #pragma unroll
-            for (int k = 0; k < kModeChunk; ++k) ar[k] = ai[k] = 0.0;
-            for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-                const double th = dq * pos[3 * j + a];
-                double s1, c1, zs, zc;
-                sincos(-th, &s1, &c1);
-                if (m0 == 0) {
-                    zs = s1;
-                    zc = c1;
-                } else {
-                    sincos(-(double)(m0 + 1) * th, &zs, &zc);
-                }
-                ar[0] += zc;
-                ai[0] += zs;
-#pragma unroll
-                for (int k = 1; k < kModeChunk; ++k) {
-                    const double nc = zc * c1 - zs * s1;
-                    const double ns = zc * s1 + zs * c1;
-                    zc = nc;
-                    zs = ns;
-                    ar[k] += zc;
-                    ai[k] += zs;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kModeChunk; ++k) {
-                const double sr = wave_sum(ar[k]);
-                const double si = wave_sum(ai[k]);
-                const int m = m0 + k;
-                if (lane == 0 && m < nq) {
-                    part[((size_t)m * nw + gw) * 6 + 2 * a] = sr;
-                    part[((size_t)m * nw + gw) * 6 + 2 * a + 1] = si;
-                }
-            }
+    for (int k = 0; k < kModeChunk; ++k) ar[k] = ai[k] = 0.0;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const double th = dq * pos[3 * j + a];
+        double s1, c1, zs, zc;
+        sincos(-th, &s1, &c1);
+        if (m0 == 0) {
+            zs = s1;
+            zc = c1;
+        } else {
+            sincos(-(double)(m0 + 1) * th, &zs, &zc);
         }
+        ar[0] += zc;
+        ai[0] += zs;
+#pragma unroll
+        for (int k = 1; k < kModeChunk; ++k) {
+            const double nc = zc * c1 - zs * s1;
+            const double ns = zc * s1 + zs * c1;
+            zc = nc;
+            zs = ns;
+            ar[k] += zc;
+            ai[k] += zs;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kModeChunk; ++k) {
+        const double sr = wave_sum(ar[k]);
+        const double si = wave_sum(ai[k]);
+        const int m = m0 + k;
+        if (lane == 0 && m < nq) {
+            part[((size_t)m * nw + gw) * 6 + 2 * a] = sr;
+            part[((size_t)m * nw + gw) * 6 + 2 * a + 1] = si;
+        }
+    }
+}
+
+// Sum the per-wave partials of k_modes_grid: block m, out[m * 6 + c].
+__global__ __launch_bounds__(256) void k_modes_grid_final(const double *__restrict__ part, int nw,
+                                                          double *__restrict__ out) {
+    __shared__ double red[4];
+    const int m = blockIdx.x;
+    for (int c = 0; c < 6; ++c) {
+        double v = 0.0;
+        for (int w = threadIdx.x; w < nw; w += blockDim.x) v += part[((size_t)m * nw + w) * 6 + c];
+        v = block_sum_256(v, red);
+        if (threadIdx.x == 0) out[(size_t)m * 6 + c] = v;
     }
 }
 
@@ -273,7 +285,7 @@ static int finalize_step(mdf_p3m_ctx *c, int n, double uf, double kf, double *re
     P.pm = c->part_pm;
     P.npm = c->n_part_pm;
     P.kp = c->part_red;
-    P.nk = c->nblock;
+    P.nk = c->brick.nblock_gather;
     P.half_inv_V = 0.5 / (c->g.L[0] * c->g.L[1] * c->g.L[2]);
     P.u_self = -(double)n * c->g.alpha / sqrt(M_PI);
     P.uf = uf;
@@ -340,12 +352,13 @@ int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int 
         return MDF_EINVAL;
     }
     int nb = (n + 255) / 256;
-    if (nb > 256) nb = 256;
+    if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
-    const int nwave = nb * 4;
-    hipLaunchKernelGGL(k_modes_grid, dim3(nb), dim3(256), 0, s, pos, n, dq, nq, c->part_red);
+    const int nw = nb * 4;
+    const int nchunk = (nq + kModeChunk - 1) / kModeChunk;
+    hipLaunchKernelGGL(k_modes_grid, dim3(nb, 3, nchunk), dim3(256), 0, s, pos, n, dq, nq, c->part_red);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_modes_final, dim3(nq), dim3(64), 0, s, c->part_red, nwave, nq, out);
+    hipLaunchKernelGGL(k_modes_grid_final, dim3(nq), dim3(256), 0, s, c->part_red, nw, out);
     MDF_LAUNCHED();
     return MDF_OK;
 }

@@ -53,16 +53,22 @@ struct Geom {
     double kappa, alpha, rc;
 };
 
-// Per-block LDS mesh brick for assignment / gather.
+// Mesh bricks of the assignment and gather kernels.  Each workgroup handles
+// a PM block of cb^3 reference cells; its brick covers the stencils of the
+// block's particles.
 struct Brick {
-    int B[3];         // brick extent in mesh nodes
-    int lds_assign;   // 1 if the charge brick fits LDS for the assignment
-    int lds_gather;   // 1 if the 3-field brick fits LDS for the gather
+    int B[3];          // assignment brick extent in mesh nodes
+    int Bg[3];         // gather brick extent
+    int cb_assign, cb_gather;
+    int nblock_assign, nblock_gather;
+    int lds_assign;    // 1 if the charge brick fits LDS for the assignment
+    int lds_gather;    // 1 if the 3-field brick fits LDS for the gather
 };
 
 constexpr int kPPThreads = 256;     // 4 waves share one 6x6x6 sub-cell tile
 constexpr int kPPWaves = kPPThreads / 64;
-constexpr int kPMThreads = 256;
+constexpr int kPMThreads = 512;
+constexpr int kGatherThreads = 1024;
 constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces
 constexpr int kMaxPartials = 1 << 20;
 

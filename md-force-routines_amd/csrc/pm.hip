@@ -6,10 +6,11 @@
 // (:29-48, evaluated in Horner form), same nodes ix..ix+5 wrapped once into
 // the mesh (:52-93), same acc = Z * E_p (:308-310).
 //
-// Work is organised per reference cell (the 2x2x2 sub-cell blocks of the
-// sorted particle array, whose particles are contiguous in four runs).  A
-// cell holds a few tens of particles, so each particle gets a group of eight
-// lanes and six of them take one z-plane (36 nodes) of its 6x6x6 stencil:
+// Work is organised per PM block of CB^3 reference cells (CB chosen per
+// context so the block's mesh brick fits LDS; the block's particles are
+// contiguous runs of the sorted array).  Each particle gets a group of
+// eight lanes and six of them take one z-plane (36 nodes) of its 6x6x6
+// stencil:
 // * scatter: the block's particles deposit into a private LDS brick of the
 //   mesh with LDS atomics, then the brick is added to the global mesh with
 //   FP64 atomics (one per non-empty brick node instead of 216 per particle);
@@ -51,38 +52,67 @@ __device__ __forceinline__ int wrap3(int i, int M) {
 }
 
 constexpr int kLanesPerParticle = 8;  // six used: one stencil z-plane each
-constexpr int kParticlesPerPass = kPMThreads / kLanesPerParticle;
 
-// Brick origin (mesh node) of block b along one axis: one node of margin
-// below the lowest base index a particle of the block can have.
-__device__ __forceinline__ int brick_origin(int b, double w, double h) { return (int)floor((2 * b) * w / h) - 1; }
-
-// The block's particles: four runs (hy, hz) of two x-adjacent sub-cells.
-struct BlockRuns {
-    int start[4];
-    int off[5];
+// A PM block is CB x CB x CB reference cells (fewer at the upper box faces).
+// Its particles are 4 CB^2 (or fewer) contiguous runs of the sorted array:
+// one per row of sub-cells along y and z, spanning the block in x.
+struct PMBlock {
+    int c0[3], nc[3];  // first reference cell and cell count per axis
+    int o[3];          // brick origin (mesh node) per axis
 };
 
-__device__ __forceinline__ BlockRuns block_runs(const Geom &g, const int *__restrict__ cell_start, int bx, int by,
-                                                int bz) {
-    BlockRuns R;
-    R.off[0] = 0;
+__device__ __forceinline__ PMBlock pm_block(const Geom &g, int cb, int bid) {
+    PMBlock b;
+    const int nbx = (g.nblk[0] + cb - 1) / cb, nby = (g.nblk[1] + cb - 1) / cb;
+    const int id[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int hy = r & 1, hz = r >> 1;
-        const int c0 = 2 * bx + g.nsub[0] * ((2 * by + hy) + g.nsub[1] * (2 * bz + hz));
-        R.start[r] = cell_start[c0];
-        R.off[r + 1] = R.off[r] + (cell_start[c0 + 2] - R.start[r]);
+    for (int d = 0; d < 3; ++d) {
+        b.c0[d] = id[d] * cb;
+        b.nc[d] = min(cb, g.nblk[d] - b.c0[d]);
+        // one node of margin below the lowest base index of the block's particles
+        b.o[d] = (int)floor((2 * b.c0[d]) * g.w[d] / g.h[d]) - 1;
     }
-    return R;
+    return b;
 }
 
-__device__ __forceinline__ int run_index(const BlockRuns &R, int k) {
-    int s = R.start[0] + k;
+// Fill the run table (start in the sorted array, prefix offsets) in LDS;
+// returns the number of runs.  Call with all threads; ends with a barrier.
+__device__ int pm_runs(const Geom &g, const int *__restrict__ cell_start, const PMBlock &b, int *run_start,
+                       int *run_off) {
+    const int ny = 2 * b.nc[1], nz = 2 * b.nc[2];
+    const int nrun = ny * nz;
+    const int t = threadIdx.x;
+    int len = 0;
+    if (t < nrun) {
+        const int sy = 2 * b.c0[1] + t % ny, sz = 2 * b.c0[2] + t / ny;
+        const int cfirst = 2 * b.c0[0] + g.nsub[0] * (sy + g.nsub[1] * sz);
+        const int s0 = cell_start[cfirst];
+        run_start[t] = s0;
+        len = cell_start[cfirst + 2 * b.nc[0]] - s0;
+    }
+    if (t < 64) {  // nrun <= 64: one wave scans the lengths
+        int incl = len;
 #pragma unroll
-    for (int r = 1; r < 4; ++r)
-        if (k >= R.off[r]) s = R.start[r] + (k - R.off[r]);
-    return s;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if (t >= o) incl += u;
+        }
+        run_off[t + 1] = incl;
+        if (t == 0) run_off[0] = 0;
+    }
+    __syncthreads();
+    return nrun;
+}
+
+// Sorted index of the block's k-th particle (binary search of the runs).
+__device__ __forceinline__ int pm_particle(const int *run_start, const int *run_off, int nrun, int k) {
+    int lo = 0, hi = nrun;  // run_off[lo] <= k < run_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (run_off[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    return run_start[lo] + (k - run_off[lo]);
 }
 
 struct AssignArgs {
@@ -97,24 +127,19 @@ struct AssignArgs {
 template <bool LDS>
 __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
     extern __shared__ double brick[];
+    __shared__ int run_start[64], run_off[65];
     const Geom &g = P.g;
     const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
     const int nb = B0 * B1 * B2;
-    const int bid = blockIdx.x;
-    const int bx = bid % g.nblk[0];
-    const int by = (bid / g.nblk[0]) % g.nblk[1];
-    const int bz = bid / (g.nblk[0] * g.nblk[1]);
-    const int o0 = brick_origin(bx, g.w[0], g.h[0]);
-    const int o1 = brick_origin(by, g.w[1], g.h[1]);
-    const int o2 = brick_origin(bz, g.w[2], g.h[2]);
+    const PMBlock blk = pm_block(g, P.br.cb_assign, blockIdx.x);
+    const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     for (int t = threadIdx.x; t < nb; t += blockDim.x) brick[t] = 0.0;
-    const BlockRuns R = block_runs(g, P.cell_start, bx, by, bz);
-    __syncthreads();
-
+    const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
+    const int npart = run_off[nrun];
     const int sub = threadIdx.x % kLanesPerParticle;  // stencil z-plane of this lane
-    for (int k = threadIdx.x / kLanesPerParticle; k < R.off[4]; k += kParticlesPerPass) {
+    for (int k = threadIdx.x / kLanesPerParticle; k < npart; k += blockDim.x / kLanesPerParticle) {
         if (sub >= 6) continue;
-        const double4 p = P.spos[run_index(R, k)];
+        const double4 p = P.spos[pm_particle(run_start, run_off, nrun, k)];
         double wx[6], wy[6], wz[6];
         const int ix = base_and_weights(p.x, g.h[0], wx);
         const int iy = base_and_weights(p.y, g.h[1], wy);
@@ -173,9 +198,9 @@ int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
     if (c->brick.lds_assign) {
         const size_t lds = sizeof(double) * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
         MDF_TRY(allow_lds(k_assign<true>, lds));
-        hipLaunchKernelGGL(k_assign<true>, dim3(c->nblock), dim3(kPMThreads), lds, s, P);
+        hipLaunchKernelGGL(k_assign<true>, dim3(c->brick.nblock_assign), dim3(kPMThreads), lds, s, P);
     } else {
-        hipLaunchKernelGGL(k_assign<false>, dim3(c->nblock), dim3(kPMThreads), 0, s, P);
+        hipLaunchKernelGGL(k_assign<false>, dim3(c->brick.nblock_assign), dim3(kPMThreads), 0, s, P);
     }
     MDF_LAUNCHED();
     if (rho_out) MDF_HIP(hipMemcpyAsync(rho_out, c->rho, c->nmesh * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -197,18 +222,14 @@ struct GatherArgs {
 };
 
 template <bool LDS>
-__global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
+__global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
     extern __shared__ double ebrick[];
-    __shared__ double red[kPMThreads / 64];
+    __shared__ double red[kGatherThreads / 64];
+    __shared__ int run_start[64], run_off[65];
     const Geom &g = P.g;
-    const int B0 = P.br.B[0], B1 = P.br.B[1], B2 = P.br.B[2];
-    const int bid = blockIdx.x;
-    const int bx = bid % g.nblk[0];
-    const int by = (bid / g.nblk[0]) % g.nblk[1];
-    const int bz = bid / (g.nblk[0] * g.nblk[1]);
-    const int o0 = brick_origin(bx, g.w[0], g.h[0]);
-    const int o1 = brick_origin(by, g.w[1], g.h[1]);
-    const int o2 = brick_origin(bz, g.w[2], g.h[2]);
+    const int B0 = P.br.Bg[0], B1 = P.br.Bg[1], B2 = P.br.Bg[2];
+    const PMBlock blk = pm_block(g, P.br.cb_gather, blockIdx.x);
+    const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     if (LDS) {
         const int nb = B0 * B1 * B2;
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
@@ -219,16 +240,17 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
             ebrick[3 * t + 1] = P.emesh[3 * node + 1];
             ebrick[3 * t + 2] = P.emesh[3 * node + 2];
         }
-        __syncthreads();
     }
-    const BlockRuns R = block_runs(g, P.cell_start, bx, by, bz);
+    const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
+    const int npart = run_off[nrun];
     double ksum = 0.0;
     const int sub = threadIdx.x % kLanesPerParticle;  // stencil z-plane of this lane
-    const int npass = (R.off[4] + kParticlesPerPass - 1) / kParticlesPerPass;
+    const int per_pass = blockDim.x / kLanesPerParticle;
+    const int npass = (npart + per_pass - 1) / per_pass;
     for (int pass = 0; pass < npass; ++pass) {
-        const int k = pass * kParticlesPerPass + threadIdx.x / kLanesPerParticle;
-        const bool live = k < R.off[4];
-        const int s = live ? run_index(R, k) : 0;
+        const int k = pass * per_pass + threadIdx.x / kLanesPerParticle;
+        const bool live = k < npart;
+        const int s = live ? pm_particle(run_start, run_off, nrun, k) : 0;
         double ex = 0.0, ey = 0.0, ez = 0.0;
         double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
         if (live) {
@@ -338,8 +360,8 @@ __global__ __launch_bounds__(kPMThreads) void k_gather(GatherArgs P) {
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0.0;
-            for (int w = 0; w < kPMThreads / 64; ++w) t += red[w];
-            P.part[bid] = t;
+            for (int w = 0; w < kGatherThreads / 64; ++w) t += red[w];
+            P.part[blockIdx.x] = t;
         }
     }
 }
@@ -357,11 +379,11 @@ int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
     P.part = c->part_red;
     P.o = o;
     if (c->brick.lds_gather) {
-        const size_t lds = 3 * sizeof(double) * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
+        const size_t lds = 3 * sizeof(double) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2];
         MDF_TRY(allow_lds(k_gather<true>, lds));
-        hipLaunchKernelGGL(k_gather<true>, dim3(c->nblock), dim3(kPMThreads), lds, s, P);
+        hipLaunchKernelGGL(k_gather<true>, dim3(c->brick.nblock_gather), dim3(kGatherThreads), lds, s, P);
     } else {
-        hipLaunchKernelGGL(k_gather<false>, dim3(c->nblock), dim3(kPMThreads), 0, s, P);
+        hipLaunchKernelGGL(k_gather<false>, dim3(c->brick.nblock_gather), dim3(kGatherThreads), 0, s, P);
     }
     MDF_LAUNCHED();
     return MDF_OK;
