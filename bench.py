@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Throughput of the P3M Yukawa NVE loop on one MI355X.
+"""Throughput of the P3M Yukawa NVE loop on one MI355X: chosen vs plain form.
 
 Workload (default): N = 1,000,000 unit-charge ions, box L = (4 pi N / 3)^(1/3),
 Gamma = 100 (T = 0.01), kappa = 2, 128^3 mesh, order-6 assignment, optimal
@@ -10,10 +10,17 @@ RMS force error of at most 1e-5 relative to the mean force; the error against
 a tight FP64 reference (alpha 0.7, rc 9, 512^3) is measured at setup and
 reported.
 
-Timed region: `steps` fused NVE steps (kick-drift, binning, PP, assignment,
-FFT solve, gather + kick, FP64 reductions, n(q, t)) with every input already
-in HBM, bracketed by synchronisations; HIP events on the kernels' stream give
-the per-stage durations that feed the roofline entry.
+Two forms run the same workload from the same seeded state:
+  plain  - FP64 everywhere (the reference's arithmetic);
+  chosen - FP32 PP pair math, assignment weights and FFT/Green/ik passes,
+           FP64 charge-mesh atomics, energy/momentum reductions and positions.
+`value` is the chosen form's throughput; the plain form's throughput, the
+ratio and both force errors are reported in `config`.
+
+Timed region (per form): `steps` fused NVE steps (kick-drift, binning, PP,
+assignment, FFT solve, gather + kick, FP64 reductions, n(q, t)) with every
+input already in HBM, bracketed by synchronisations; HIP events on the
+kernels' stream give the per-stage durations that feed the roofline entry.
 
 Prints one JSON line.
 """
@@ -34,6 +41,7 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at RMS force err<=1e-5, chosen vs plain FP64 P3M, 1 GPU"
 PEAK_HBM = 8.0e12          # B/s, MI355X
 PEAK_FP64 = 78.6e12        # FLOP/s, MI355X dense FP64
+PEAK_FP32 = 157.3e12       # FLOP/s, MI355X dense FP32
 
 
 def parse():
@@ -52,6 +60,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-err", action="store_true", help="skip the force-error measurement")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both",
+                    help="forms to time; value is the chosen form's unless only plain runs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -100,73 +112,106 @@ def main():
     T = 1.0 / a.gamma
     pos0, vel0 = initial_split(1, 2, N, [L, L, L], T)
 
-    eng = P3M(N, M, L, a.kappa, a.alpha, a.rc, device=dev)
-    G_full, _ = eng.build_green(6, 3, want_full=True)
-    pos = torch.from_numpy(pos0).to(dev)
-    vel = torch.from_numpy(vel0).to(dev)
-    acc, en = eng.force(pos)
-    acc = acc.clone()
-    torch.cuda.synchronize()
-    acc0 = acc.cpu().numpy()
+    dq = 2.0 * math.pi / L
+    if a.calib_copy:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # reads and writes 1 GiB each
+        torch.cuda.synchronize()
+        del src, dst
 
-    err = {}
+    acc_t = None
     if not a.no_err:
         ref = P3M(N, 512, L, a.kappa, 0.7, 9.0, device=dev)
         ref.build_green(6, 3)
-        acc_t, _ = ref.force(pos)
+        acc_t, _ = ref.force(torch.from_numpy(pos0).to(dev))
+        acc_t = acc_t.clone()
+        ref.close()
+        del ref
+        torch.cuda.synchronize()
+
+    def force_error(acc):
+        if acc_t is None:
+            return {}
         d = acc - acc_t
         rms_d = float(torch.sqrt(torch.mean(torch.sum(d * d, 1))))
         rms_f = float(torch.sqrt(torch.mean(torch.sum(acc_t * acc_t, 1))))
         mean_f = float(torch.mean(torch.sqrt(torch.sum(acc_t * acc_t, 1))))
-        err = {"rms_rel_rms": rms_d / rms_f, "rms_rel_meanF": rms_d / mean_f,
-               "reference": "FP64 P3M alpha 0.7, rc 9, 512^3"}
-        ref.close()
-        del acc_t, d
+        return {"rms_rel_rms": rms_d / rms_f, "rms_rel_meanF": rms_d / mean_f,
+                "reference": "FP64 P3M alpha 0.7, rc 9, 512^3"}
+
+    def run_form(precision):
+        eng = P3M(N, M, L, a.kappa, a.alpha, a.rc, precision=precision, device=dev)
+        G_full, _ = eng.build_green(6, 3, want_full=True)
+        pos = torch.from_numpy(pos0).to(dev)
+        vel = torch.from_numpy(vel0).to(dev)
+        acc, _ = eng.force(pos)
+        acc = acc.clone()
         torch.cuda.synchronize()
+        err = force_error(acc)
+        acc_init = acc.cpu().numpy() if precision == "fp64" else None
+        if a.warmup:
+            eng.md_run(pos, vel, acc, a.warmup, a.dt, q_step=dq, nq=a.nq)
+        energies = torch.empty((a.steps, 4), dtype=torch.float64, device=dev)
+        nqt = torch.empty((a.steps, a.nq, 3), dtype=torch.complex128, device=dev)
+        torch.cuda.synchronize()
+        eng.profile_start(events=16 * (a.steps + 4))
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record()
+        eng.md_run(pos, vel, acc, a.steps, a.dt, q_step=dq, nq=a.nq, energies=energies, nqt=nqt)
+        stop.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gpu_ms = start.elapsed_time(stop)
+        stages = eng.profile_read()
+        eng.profile_stop()
+        en_h = energies.cpu().numpy()
+        drift = float(np.max(np.abs(en_h[:, 0] - en_h[0, 0])) / (N * T))
+        eng.close()
+        return {"gpu_ms": gpu_ms, "wall": wall, "stages": stages, "err": err, "drift": drift, "G_full": G_full,
+                "acc0": acc_init, "value": N * a.steps / (gpu_ms / 1e3)}
 
-    dq = 2.0 * math.pi / L
-    q = torch.arange(1, a.nq + 1, dtype=torch.float64, device=dev) * dq
-    Z = None
-    # warm-up
-    if a.warmup:
-        eng.md_run(pos, vel, acc, a.warmup, a.dt, Z=Z, q_step=dq, nq=a.nq)
-    energies = torch.empty((a.steps, 4), dtype=torch.float64, device=dev)
-    nqt = torch.empty((a.steps, a.nq, 3), dtype=torch.complex128, device=dev)
-    torch.cuda.synchronize()
-    eng.profile_start(events=16 * (a.steps + 4))
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    eng.md_run(pos, vel, acc, a.steps, a.dt, Z=Z, q_step=dq, nq=a.nq, energies=energies, nqt=nqt)
-    stop.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(stop)
-    stages = eng.profile_read()
-    eng.profile_stop()
-    ms_per_step = gpu_ms / a.steps
-    value = N * a.steps / (gpu_ms / 1e3)
+    forms = {"both": ("fp64", "mixed"), "chosen": ("mixed",), "plain": ("fp64",)}[a.form]
+    res = {f: run_form(f) for f in forms}
+    main_form = "mixed" if "mixed" in res else "fp64"
+    R = res[main_form]
+    value = R["value"]
+    ms_per_step = R["gpu_ms"] / a.steps
+    stages = R["stages"]
 
-    en_h = energies.cpu().numpy()
-    drift = float(np.max(np.abs(en_h[:, 0] - en_h[0, 0])) / (N * T))
-
-    # roofline of the dominant stage
+    # roofline of the dominant stage of the reported form
     dom = max((s for s in stages if s != "finalize"), key=lambda s: stages[s][0])
     tot_ms, cnt = stages[dom]
     avg_s = tot_ms / max(cnt, 1) / 1e3
     by, fl = stage_work(dom, N, M, a.rc)
     if fl > 0:
-        roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+        peak = PEAK_FP32 if main_form == "mixed" else PEAK_FP64
+        roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
     else:
         roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
-    roof["kernel"] = {"pp": "k_pp_fp64", "fft": "fft passes", "gather": "k_gather", "assign": "k_assign",
-                      "bin": "bin kernels", "drift": "k_kick_drift", "modes": "k_modes"}.get(dom, dom)
+    pp_name = "k_pp<float>" if main_form == "mixed" else "k_pp<double>"
+    roof["kernel"] = {"pp": pp_name, "fft": "fft passes", "gather": "k_gather", "assign": "k_assign",
+                      "bin": "bin kernels", "drift": "k_kick_drift", "modes": "k_modes_grid"}.get(dom, dom)
     roof["avg_launch_ms"] = avg_s * 1e3
-    stage_ms = {s: round(v[0] / max(v[1], 1), 4) for s, v in stages.items() if v[1]}
+
+    def stage_ms(st):
+        return {s: round(v[0] / max(v[1], 1), 4) for s, v in st.items() if v[1]}
+
+    form_cfg = {}
+    for f, r in res.items():
+        form_cfg["chosen" if f == "mixed" else "plain"] = {
+            "particle_steps_per_s": r["value"], "ms_per_step": r["gpu_ms"] / a.steps, "force_error": r["err"],
+            "energy_drift_per_NT": r["drift"], "stage_ms": stage_ms(r["stages"]), "wall_s": r["wall"]}
+    if len(res) == 2:
+        form_cfg["chosen_over_plain"] = res["mixed"]["value"] / res["fp64"]["value"]
+    G_full = R["G_full"]
+    acc0 = res["fp64"]["acc0"] if "fp64" in res else None
+    q = torch.arange(1, a.nq + 1, dtype=torch.float64, device=dev) * dq
 
     cpu = None
     if not a.no_cpu:
@@ -174,7 +219,12 @@ def main():
         c_oracle.build(ref=False)
         Gh = G_full.cpu().numpy()
         p_h, v_h = pos0.copy(), vel0.copy()
-        a_h = acc0.copy()  # the initial force (FP64, equal to the host's to 1e-12)
+        if acc0 is None:  # initial FP64 force, equal to the C restatement's to 1e-12
+            e64 = P3M(N, M, L, a.kappa, a.alpha, a.rc, device=dev)
+            e64.build_green(6, 3)
+            acc0 = e64.force(torch.from_numpy(pos0).to(dev))[0].cpu().numpy()
+            e64.close()
+        a_h = acc0.copy()
         qh = q.cpu().numpy()
         t0 = time.perf_counter()
         c_oracle.md_steps(p_h, v_h, a_h, np.ones(N), a.cpu_steps, a.dt, L, a.kappa, a.alpha, a.rc, Gh, M, q=qh)
@@ -194,14 +244,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if main_form == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP64 P3M Yukawa OCP NVE, N={N}, {M}^3 mesh (Gamma={a.gamma:g}, kappa={a.kappa:g})",
-            "form": "plain",
+            "workload": f"P3M Yukawa OCP NVE, N={N}, {M}^3 mesh (Gamma={a.gamma:g}, kappa={a.kappa:g}), "
+                        f"{'FP32-mixed' if main_form == 'mixed' else 'plain FP64'}",
+            "form": "chosen (FP32-mixed)" if main_form == "mixed" else "plain",
             "N": N, "mesh": M, "L": L, "kappa": a.kappa, "Gamma": a.gamma, "alpha": a.alpha, "rc": a.rc,
-            "dt": a.dt, "nq": a.nq, "force_error": err, "energy_drift_per_NT": drift,
-            "stage_ms": stage_ms, "wall_s": wall, "parallelism": "single GPU",
+            "dt": a.dt, "nq": a.nq, "forms": form_cfg, "parallelism": "single GPU",
         },
         "roofline": roof,
         "cpu_baseline": cpu,

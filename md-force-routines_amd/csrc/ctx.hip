@@ -79,10 +79,11 @@ static int dmalloc(T **p, size_t count) {
 static void free_ctx(mdf_p3m_ctx *c) {
     if (!c) return;
     prof_free(c);
-    void *ptrs[] = {c->cheb, c->scan_tmp, c->cell_count, c->cell_start, c->part_cell, c->part_slot, c->perm,    c->spos,
-                    c->acc_pp_s,   c->rho,        c->ck,        c->ek,        c->emesh,   c->green,
-                    c->tw[0],      c->tw[1],      c->tw[2],     c->part_pp,   c->part_pm, c->part_red,
-                    c->scal,       c->stats};
+    void *ptrs[] = {c->cheb,     c->chebf,     c->scan_tmp, c->cell_count, c->cell_start, c->part_cell,
+                    c->part_slot, c->perm,      c->spos,     c->acc_pp_s,   c->rho,        c->ck,
+                    c->ek,        c->emesh,     c->green,    c->tw[0],      c->tw[1],      c->tw[2],
+                    c->ckf,       c->ekf,       c->emeshf,   c->greenf,     c->twf[0],     c->twf[1],
+                    c->twf[2],    c->part_pp,   c->part_pm,  c->part_red,   c->scal,       c->stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -124,8 +125,8 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         set_error("charge-assignment order %d not supported (only 6, as yukawa_pm_ord6)", p->order);
         return MDF_EINVAL;
     }
-    if (p->precision != MDF_PREC_FP64) {
-        set_error("precision mode %d not available in this build", p->precision);
+    if (p->precision != MDF_PREC_FP64 && p->precision != MDF_PREC_MIXED) {
+        set_error("precision mode %d unknown (MDF_PREC_FP64 = 0, MDF_PREC_MIXED = 1)", p->precision);
         return MDF_EINVAL;
     }
     if (p->n_max <= 0) {
@@ -183,6 +184,8 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         set_error("too many cells (%d > %d): increase rc", c->nblock, kMaxPartials);
         return MDF_EINVAL;
     }
+    const bool mixed = p->precision == MDF_PREC_MIXED;
+    const size_t es = mixed ? 4 : 8;  // bytes of the pair/mesh arithmetic type
     c->nmesh = (size_t)g.M[0] * g.M[1] * g.M[2];
     c->nspec = (size_t)g.M[2] * g.M[1] * g.Mxh;
 
@@ -196,16 +199,17 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     c->tile_max = ((int)tm + 3) & ~3;  // keeps the table after the tile 16-byte aligned
     std::vector<double2> pair_tab;
     {
-        const int rr = build_pair_table(p->kappa, p->alpha, p->rc, pair_tab, c->npiece);
+        const int rr = mixed ? build_pair_table(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, true, pair_tab, c->npiece)
+                             : build_pair_table(p->kappa, p->alpha, p->rc, kChebDeg, 4e-15, false, pair_tab, c->npiece);
         if (rr != MDF_OK) {
             delete c;
             return rr;
         }
     }
-    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece);
+    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed);
     if (c->pp_lds > 160 * 1024) {
         c->tile_max -= (int)((c->pp_lds - 160 * 1024) / 24 + 4) & ~3;
-        c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece);
+        c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed);
     }
 
     // PM blocks: the largest cb (<= 4 reference cells per axis) whose brick
@@ -232,6 +236,8 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         c->brick.cb_gather = 1;
         for (int cb = 4; cb >= 1; --cb) {
             int B[3];
+            // sized by node count (8192 nodes) in both precisions: a larger
+            // block would leave too few workgroups to fill the chip
             if (extent(cb, B) * 8 <= 64 * 1024 || cb == 1) {
                 c->brick.cb_assign = cb;
                 for (int d = 0; d < 3; ++d) c->brick.B[d] = B[d];
@@ -241,10 +247,10 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         }
         for (int cb = 4; cb >= 1; --cb) {
             int B[3];
-            if (extent(cb, B) * 24 <= 144 * 1024 || cb == 1) {
+            if (extent(cb, B) * 3 * es <= 144 * 1024 || cb == 1) {
                 c->brick.cb_gather = cb;
                 for (int d = 0; d < 3; ++d) c->brick.Bg[d] = B[d];
-                c->brick.lds_gather = extent(cb, B) * 24 <= 144 * 1024 ? 1 : 0;
+                c->brick.lds_gather = extent(cb, B) * 3 * es <= 144 * 1024 ? 1 : 0;
                 break;
             }
         }
@@ -263,13 +269,22 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     ALLOC(c->perm, (size_t)p->n_max);
     ALLOC(c->spos, (size_t)p->n_max);
     ALLOC(c->acc_pp_s, 3 * (size_t)p->n_max);
-    ALLOC(c->cheb, pair_tab.size());
     ALLOC(c->rho, c->nmesh);
-    ALLOC(c->ck, c->nspec);
-    ALLOC(c->ek, 3 * c->nspec);
-    ALLOC(c->emesh, 3 * c->nmesh);
     ALLOC(c->green, c->nspec);
-    for (int d = 0; d < 3; ++d) ALLOC(c->tw[d], (size_t)g.M[d] / 2);
+    if (mixed) {
+        ALLOC(c->chebf, pair_tab.size());
+        ALLOC(c->ckf, c->nspec);
+        ALLOC(c->ekf, 3 * c->nspec);
+        ALLOC(c->emeshf, 3 * c->nmesh);
+        ALLOC(c->greenf, c->nspec);
+        for (int d = 0; d < 3; ++d) ALLOC(c->twf[d], (size_t)g.M[d] / 2);
+    } else {
+        ALLOC(c->cheb, pair_tab.size());
+        ALLOC(c->ck, c->nspec);
+        ALLOC(c->ek, 3 * c->nspec);
+        ALLOC(c->emesh, 3 * c->nmesh);
+        for (int d = 0; d < 3; ++d) ALLOC(c->tw[d], (size_t)g.M[d] / 2);
+    }
     ALLOC(c->part_pp, (size_t)c->nblock);
     ALLOC(c->part_pm, (size_t)g.M[1] * (g.Mxh + 3));
     ALLOC(c->part_red, (size_t)kMaxPartials);
@@ -280,8 +295,15 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         free_ctx(c);
         return r;
     }
-    if (hipMemcpy(c->cheb, pair_tab.data(), pair_tab.size() * sizeof(double2), hipMemcpyHostToDevice) !=
-        hipSuccess) {
+    hipError_t up;
+    if (mixed) {
+        std::vector<float2> tf(pair_tab.size());
+        for (size_t i = 0; i < tf.size(); ++i) tf[i] = make_float2((float)pair_tab[i].x, (float)pair_tab[i].y);
+        up = hipMemcpy(c->chebf, tf.data(), tf.size() * sizeof(float2), hipMemcpyHostToDevice);
+    } else {
+        up = hipMemcpy(c->cheb, pair_tab.data(), pair_tab.size() * sizeof(double2), hipMemcpyHostToDevice);
+    }
+    if (up != hipSuccess) {
         set_error("pair-table upload failed");
         free_ctx(c);
         return MDF_EHIP;
@@ -290,11 +312,15 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     for (int d = 0; d < 3; ++d) {
         const int m = g.M[d];
         std::vector<double2> tw(m / 2);
+        std::vector<float2> twf(m / 2);
         for (int t = 0; t < m / 2; ++t) {
             const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)m;
             tw[t] = make_double2((double)cosl(ang), (double)-sinl(ang));
+            twf[t] = make_float2((float)cosl(ang), (float)-sinl(ang));
         }
-        if (hipMemcpy(c->tw[d], tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        const hipError_t e = mixed ? hipMemcpy(c->twf[d], twf.data(), twf.size() * sizeof(float2), hipMemcpyHostToDevice)
+                                   : hipMemcpy(c->tw[d], tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
             set_error("twiddle upload failed");
             free_ctx(c);
             return MDF_EHIP;

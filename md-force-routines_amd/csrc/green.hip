@@ -143,7 +143,7 @@ int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, d
         MDF_LAUNCHED();
     }
     c->have_green = true;
-    return MDF_OK;
+    return green_to_float(c, s);
 }
 
 int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s) {
@@ -152,6 +152,21 @@ int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s) {
                        c->g.M[0], c->g.M[1], c->g.M[2], c->g.Mxh);
     MDF_LAUNCHED();
     c->have_green = true;
+    return green_to_float(c, s);
+}
+
+__global__ void k_to_float(const double *__restrict__ src, float *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+int green_to_float(mdf_p3m_ctx *c, hipStream_t s) {
+    if (!c->greenf) return MDF_OK;
+    const size_t n = c->nspec;
+    size_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)nb), dim3(256), 0, s, c->green, c->greenf, n);
+    MDF_LAUNCHED();
     return MDF_OK;
 }
 

@@ -69,7 +69,8 @@ constexpr int kPPThreads = 256;     // 4 waves share one 6x6x6 sub-cell tile
 constexpr int kPPWaves = kPPThreads / 64;
 constexpr int kPMThreads = 512;
 constexpr int kGatherThreads = 1024;
-constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces
+constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces (FP64)
+constexpr int kChebDegF = 7;        // degree of the FP32 pieces (mixed precision)
 constexpr int kMaxPartials = 1 << 20;
 
 // ctx->stats (device ints): [0] PP tiles needing more than one LDS pass,
@@ -101,6 +102,7 @@ struct mdf_p3m_ctx {
     double *acc_pp_s;   // sorted PP acceleration (n, 3)
     double2 *cheb;      // PP pair-term tables, npiece * (kChebDeg + 1) (g1, g2) pairs
     int npiece;
+    float2 *chebf;      // FP32 tables of the mixed mode, npiece * (kChebDegF + 1)
 
     // mesh
     double *rho;        // nmesh
@@ -109,6 +111,12 @@ struct mdf_p3m_ctx {
     double *emesh;      // 3 * nmesh, interleaved (Ex, Ey, Ez) per node
     double *green;      // nspec, natural order half spectrum
     double2 *tw[3];     // twiddles per axis, M/2 entries
+    // FP32 copies of the mixed mode (MDF_PREC_MIXED); the FP64 ck/ek/emesh are
+    // then not allocated
+    float2 *ckf, *ekf;
+    float *emeshf;
+    float *greenf;
+    float2 *twf[3];
 
     // reductions
     double *part_pp;    // per-block PP energy partials (nblock)
@@ -163,8 +171,11 @@ int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, d
 int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
 
 // PP pair-term tables (host) and the PP kernel's dynamic LDS size.
-int build_pair_table(double kappa, double alpha, double rc, std::vector<double2> &tab, int &npiece);
-size_t pp_lds_bytes(int tile_max, int npiece);
+int build_pair_table(double kappa, double alpha, double rc, int deg, double tol, bool as_float,
+                     std::vector<double2> &tab, int &npiece);
+size_t pp_lds_bytes(int tile_max, int npiece, bool mixed);
+// FP32 copy of the FP64 half-spectrum Green function (mixed mode).
+int green_to_float(mdf_p3m_ctx *c, hipStream_t s);
 
 // Fill helpers (kernels, so they can live inside captured graphs).
 int fill_zero(double *p, size_t n, hipStream_t s);

@@ -2,11 +2,13 @@
 //
 // g1(r) = r phi(r) and g2(r) = r^2 f(r) of the reference pair terms
 // (yukawa_pp.py:143-152) are fitted on [0, rc] by piecewise Chebyshev
-// series: npiece equal pieces, degree kChebDeg, coefficients from the
+// series: npiece equal pieces of a given degree, coefficients from the
 // discrete cosine transform of long-double values at the Chebyshev nodes of
-// each piece.  The fit is then checked against direct long-double
-// evaluation at 64 points per piece; if the worst error exceeds 4e-15 of
-// max|g| on [0, rc], the piece count is doubled (up to 512).
+// each piece.  The fit, with its coefficients rounded to the storage type
+// (FP64, or FP32 for the mixed-precision kernel), is then checked against
+// direct long-double evaluation at 64 points per piece; if the worst error
+// exceeds tol * max|g| on [0, rc], the piece count is doubled (up to 512).
+// FP64 tables use degree 13 and tol 4e-15; FP32 tables degree 7 and 1e-7.
 #include <cmath>
 #include <vector>
 
@@ -24,19 +26,28 @@ static void pair_g(long double r, long double kappa, long double alpha, long dou
          (2.0L * alpha / sqrtl(pi)) * r * expl(-(alpha * r) * (alpha * r) - b * b);
 }
 
-static long double clenshaw_ld(const double2 *c, int comp, long double t) {
+static long double clenshaw_ld(const double2 *c, int deg, int comp, bool as_float, long double t) {
     long double b1 = 0.0L, b2 = 0.0L;
-    for (int k = kChebDeg; k >= 1; --k) {
-        const long double ck = comp ? c[k].y : c[k].x;
+    auto coef = [&](int k) -> long double {
+        const double v = comp ? c[k].y : c[k].x;
+        return as_float ? (long double)(float)v : (long double)v;
+    };
+    for (int k = deg; k >= 1; --k) {
+        const long double ck = coef(k);
         const long double nb = 2.0L * t * b1 - b2 + ck;
         b2 = b1;
         b1 = nb;
     }
-    return t * b1 - b2 + (comp ? c[0].y : c[0].x);
+    return t * b1 - b2 + coef(0);
 }
 
-int build_pair_table(double kappa, double alpha, double rc, std::vector<double2> &tab, int &npiece) {
-    const int nc = kChebDeg + 1;
+int build_pair_table(double kappa, double alpha, double rc, int deg, double tol, bool as_float,
+                     std::vector<double2> &tab, int &npiece) {
+    const int nc = deg + 1;
+    if (nc > 32) {
+        set_error("pair-table degree %d too large", deg);
+        return MDF_EINVAL;
+    }
     const long double pi = 3.141592653589793238462643383279502884L;
     double scale = alpha * rc;
     if (0.5 * kappa * rc > scale) scale = 0.5 * kappa * rc;
@@ -46,7 +57,7 @@ int build_pair_table(double kappa, double alpha, double rc, std::vector<double2>
         const long double w = (long double)rc / P;
         tab.assign((size_t)P * nc, make_double2(0.0, 0.0));
         for (int p = 0; p < P; ++p) {
-            long double v1[kChebDeg + 1], v2[kChebDeg + 1];
+            long double v1[32], v2[32];
             for (int j = 0; j < nc; ++j) {
                 const long double t = cosl(pi * (j + 0.5L) / nc);
                 pair_g(p * w + (t + 1.0L) * 0.5L * w, kappa, alpha, v1[j], v2[j]);
@@ -70,19 +81,19 @@ int build_pair_table(double kappa, double alpha, double rc, std::vector<double2>
                 long double g1, g2;
                 pair_g(p * w + (t + 1.0L) * 0.5L * w, kappa, alpha, g1, g2);
                 const double2 *c = &tab[(size_t)p * nc];
-                const long double a1 = clenshaw_ld(c, 0, t), a2 = clenshaw_ld(c, 1, t);
+                const long double a1 = clenshaw_ld(c, deg, 0, as_float, t), a2 = clenshaw_ld(c, deg, 1, as_float, t);
                 m1 = fmaxl(m1, fabsl(g1));
                 m2 = fmaxl(m2, fabsl(g2));
                 e1 = fmaxl(e1, fabsl(a1 - g1));
                 e2 = fmaxl(e2, fabsl(a2 - g2));
             }
         }
-        if (e1 <= 4e-15L * m1 && e2 <= 4e-15L * m2) {
+        if (e1 <= (long double)tol * m1 && e2 <= (long double)tol * m2) {
             npiece = P;
             return MDF_OK;
         }
     }
-    set_error("pair-term table did not reach 4e-15 with 512 pieces (kappa=%g alpha=%g rc=%g)", kappa, alpha, rc);
+    set_error("pair-term table did not reach %g with 512 pieces (kappa=%g alpha=%g rc=%g)", tol, kappa, alpha, rc);
     return MDF_EINVAL;
 }
 

@@ -25,20 +25,24 @@
 
 namespace mdf {
 
-// Weights of yukawa_pm_ord6.py:29-48 in Horner form.
-__device__ __forceinline__ void weights6(double x, double w[6]) {
-    w[0] = (1.0 + x * (-10.0 + x * (40.0 + x * (-80.0 + x * (80.0 + x * -32.0))))) * (1.0 / 3840.0);
-    w[1] = (237.0 + x * (-750.0 + x * (840.0 + x * (-240.0 + x * (-240.0 + x * 160.0))))) * (1.0 / 3840.0);
-    w[2] = (841.0 + x * (-770.0 + x * (-440.0 + x * (560.0 + x * (80.0 + x * -160.0))))) * (1.0 / 1920.0);
-    w[3] = (841.0 + x * (770.0 + x * (-440.0 + x * (-560.0 + x * (80.0 + x * 160.0))))) * (1.0 / 1920.0);
-    w[4] = (237.0 + x * (750.0 + x * (840.0 + x * (240.0 + x * (-240.0 + x * -160.0))))) * (1.0 / 3840.0);
-    w[5] = (1.0 + x * (10.0 + x * (40.0 + x * (80.0 + x * (80.0 + x * 32.0))))) * (1.0 / 3840.0);
+// Weights of yukawa_pm_ord6.py:29-48 in Horner form, in arithmetic T.
+template <class T>
+__device__ __forceinline__ void weights6(T x, T w[6]) {
+    w[0] = ((T)1 + x * ((T)-10 + x * ((T)40 + x * ((T)-80 + x * ((T)80 + x * (T)-32))))) * (T)(1.0 / 3840.0);
+    w[1] = ((T)237 + x * ((T)-750 + x * ((T)840 + x * ((T)-240 + x * ((T)-240 + x * (T)160))))) * (T)(1.0 / 3840.0);
+    w[2] = ((T)841 + x * ((T)-770 + x * ((T)-440 + x * ((T)560 + x * ((T)80 + x * (T)-160))))) * (T)(1.0 / 1920.0);
+    w[3] = ((T)841 + x * ((T)770 + x * ((T)-440 + x * ((T)-560 + x * ((T)80 + x * (T)160))))) * (T)(1.0 / 1920.0);
+    w[4] = ((T)237 + x * ((T)750 + x * ((T)840 + x * ((T)240 + x * ((T)-240 + x * (T)-160))))) * (T)(1.0 / 3840.0);
+    w[5] = ((T)1 + x * ((T)10 + x * ((T)40 + x * ((T)80 + x * ((T)80 + x * (T)32))))) * (T)(1.0 / 3840.0);
 }
 
-__device__ __forceinline__ int base_and_weights(double x, double h, double w[6]) {
+// Base index floor(x/h) and the weights of the offset x/h - (ix + 1/2); the
+// index and offset are formed in FP64 from the FP64 position.
+template <class T>
+__device__ __forceinline__ int base_and_weights(double x, double h, T w[6]) {
     const int i0 = (int)floor(x / h);
     const double fr = (x - (i0 + 0.5) * h) / h;
-    weights6(fr, w);
+    weights6<T>((T)fr, w);
     return i0;
 }
 
@@ -124,9 +128,10 @@ struct AssignArgs {
     int *__restrict__ stats;
 };
 
-template <bool LDS>
+template <bool LDS, class T>
 __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
-    extern __shared__ double brick[];
+    extern __shared__ __align__(16) unsigned char braw[];
+    double *brick = (double *)braw;  // FP64 in both modes: FP32 LDS atomics are ~6x slower here
     __shared__ int run_start[64], run_off[65];
     const Geom &g = P.g;
     const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
@@ -140,21 +145,21 @@ __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
     for (int k = threadIdx.x / kLanesPerParticle; k < npart; k += blockDim.x / kLanesPerParticle) {
         if (sub >= 6) continue;
         const double4 p = P.spos[pm_particle(run_start, run_off, nrun, k)];
-        double wx[6], wy[6], wz[6];
-        const int ix = base_and_weights(p.x, g.h[0], wx);
-        const int iy = base_and_weights(p.y, g.h[1], wy);
-        const int iz = base_and_weights(p.z, g.h[2], wz);
+        T wx[6], wy[6], wz[6];
+        const int ix = base_and_weights<T>(p.x, g.h[0], wx);
+        const int iy = base_and_weights<T>(p.y, g.h[1], wy);
+        const int iz = base_and_weights<T>(p.z, g.h[2], wz);
         const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
         const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
-        double wza = wz[0];
+        T wza = wz[0];
 #pragma unroll
         for (int a = 1; a < 6; ++a)
             if (sub == a) wza = wz[a];
-        const double zw = p.w * wza;
+        const T zw = (T)p.w * wza;
         if (LDS && inside) {
 #pragma unroll
             for (int b = 0; b < 6; ++b) {
-                const double zwy = zw * wy[b];
+                const T zwy = zw * wy[b];
                 double *row = brick + ((rz + sub) * B1 + (ry + b)) * B0 + rx;
 #pragma unroll
                 for (int c = 0; c < 6; ++c) atomicAdd(row + c, zwy * wx[c]);
@@ -164,10 +169,10 @@ __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
             const int gz = wrap3(iz + sub, g.M[2]);
             for (int b = 0; b < 6; ++b) {
                 const int gy = wrap3(iy + b, g.M[1]);
-                const double zwy = zw * wy[b];
+                const T zwy = zw * wy[b];
                 for (int c = 0; c < 6; ++c) {
                     const int gx = wrap3(ix + c, g.M[0]);
-                    unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], zwy * wx[c]);
+                    unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], (double)(zwy * wx[c]));
                 }
             }
         }
@@ -195,12 +200,21 @@ int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
     P.cell_start = c->cell_start;
     P.rho = c->rho;
     P.stats = c->stats;
+    const bool mixed = c->prm.precision == MDF_PREC_MIXED;
+    const dim3 grid(c->brick.nblock_assign);
     if (c->brick.lds_assign) {
-        const size_t lds = sizeof(double) * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
-        MDF_TRY(allow_lds(k_assign<true>, lds));
-        hipLaunchKernelGGL(k_assign<true>, dim3(c->brick.nblock_assign), dim3(kPMThreads), lds, s, P);
+        const size_t lds = 8 * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
+        if (mixed) {
+            MDF_TRY(allow_lds(k_assign<true, float>, lds));
+            hipLaunchKernelGGL((k_assign<true, float>), grid, dim3(kPMThreads), lds, s, P);
+        } else {
+            MDF_TRY(allow_lds(k_assign<true, double>, lds));
+            hipLaunchKernelGGL((k_assign<true, double>), grid, dim3(kPMThreads), lds, s, P);
+        }
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_assign<false, float>), grid, dim3(kPMThreads), 0, s, P);
     } else {
-        hipLaunchKernelGGL(k_assign<false>, dim3(c->brick.nblock_assign), dim3(kPMThreads), 0, s, P);
+        hipLaunchKernelGGL((k_assign<false, double>), grid, dim3(kPMThreads), 0, s, P);
     }
     MDF_LAUNCHED();
     if (rho_out) MDF_HIP(hipMemcpyAsync(rho_out, c->rho, c->nmesh * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -215,15 +229,17 @@ struct GatherArgs {
     const double4 *__restrict__ spos;
     const int *__restrict__ cell_start;
     const int *__restrict__ perm;
-    const double *__restrict__ emesh;   // interleaved (Ex, Ey, Ez)
+    const void *__restrict__ emesh;     // interleaved (Ex, Ey, Ez), element type T of k_gather
     const double *__restrict__ acc_pp_s;
     double *__restrict__ part;          // K partials when kicking
     GatherOut o;
 };
 
-template <bool LDS>
+template <bool LDS, class T>
 __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
-    extern __shared__ double ebrick[];
+    extern __shared__ __align__(16) unsigned char eraw[];
+    T *ebrick = (T *)eraw;
+    const T *emesh = (const T *)P.emesh;
     __shared__ double red[kGatherThreads / 64];
     __shared__ int run_start[64], run_off[65];
     const Geom &g = P.g;
@@ -236,9 +252,9 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
             const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
             const size_t node = ((size_t)wrap3(o2 + tz, g.M[2]) * g.M[1] + wrap3(o1 + ty, g.M[1])) * g.M[0] +
                                 wrap3(o0 + tx, g.M[0]);
-            ebrick[3 * t + 0] = P.emesh[3 * node + 0];
-            ebrick[3 * t + 1] = P.emesh[3 * node + 1];
-            ebrick[3 * t + 2] = P.emesh[3 * node + 2];
+            ebrick[3 * t + 0] = emesh[3 * node + 0];
+            ebrick[3 * t + 1] = emesh[3 * node + 1];
+            ebrick[3 * t + 2] = emesh[3 * node + 2];
         }
     }
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
@@ -251,16 +267,16 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
         const int k = pass * per_pass + threadIdx.x / kLanesPerParticle;
         const bool live = k < npart;
         const int s = live ? pm_particle(run_start, run_off, nrun, k) : 0;
-        double ex = 0.0, ey = 0.0, ez = 0.0;
+        T ex = 0, ey = 0, ez = 0;
         double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
         if (live) {
             p = P.spos[s];
             if (sub < 6) {
-                double wx[6], wy[6], wz[6];
-                const int ix = base_and_weights(p.x, g.h[0], wx);
-                const int iy = base_and_weights(p.y, g.h[1], wy);
-                const int iz = base_and_weights(p.z, g.h[2], wz);
-                double wza = wz[0];
+                T wx[6], wy[6], wz[6];
+                const int ix = base_and_weights<T>(p.x, g.h[0], wx);
+                const int iy = base_and_weights<T>(p.y, g.h[1], wy);
+                const int iz = base_and_weights<T>(p.z, g.h[2], wz);
+                T wza = wz[0];
 #pragma unroll
                 for (int a = 1; a < 6; ++a)
                     if (sub == a) wza = wz[a];
@@ -269,11 +285,11 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
                 if (LDS && inside) {
 #pragma unroll
                     for (int b = 0; b < 6; ++b) {
-                        const double wzy = wza * wy[b];
-                        const double *row = ebrick + 3 * (((rz + sub) * B1 + (ry + b)) * B0 + rx);
+                        const T wzy = wza * wy[b];
+                        const T *row = ebrick + 3 * (((rz + sub) * B1 + (ry + b)) * B0 + rx);
 #pragma unroll
                         for (int c = 0; c < 6; ++c) {
-                            const double ww = wzy * wx[c];
+                            const T ww = wzy * wx[c];
                             ex += row[3 * c + 0] * ww;
                             ey += row[3 * c + 1] * ww;
                             ez += row[3 * c + 2] * ww;
@@ -283,12 +299,12 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
                     const int gz = wrap3(iz + sub, g.M[2]);
                     for (int b = 0; b < 6; ++b) {
                         const int gy = wrap3(iy + b, g.M[1]);
-                        const double wzy = wza * wy[b];
-                        const double *row = P.emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
+                        const T wzy = wza * wy[b];
+                        const T *row = emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
 #pragma unroll
                         for (int c = 0; c < 6; ++c) {
                             const int gx = wrap3(ix + c, g.M[0]);
-                            const double ww = wzy * wx[c];
+                            const T ww = wzy * wx[c];
                             ex += row[3 * gx + 0] * ww;
                             ey += row[3 * gx + 1] * ww;
                             ez += row[3 * gx + 2] * ww;
@@ -306,11 +322,11 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
         }
         if (!live || sub != 0) continue;
         const int i = P.perm[s];
-        const double apx = p.w * ex, apy = p.w * ey, apz = p.w * ez;
+        const double apx = p.w * (double)ex, apy = p.w * (double)ey, apz = p.w * (double)ez;
         const GatherOut &o = P.o;
-        if (o.ex) o.ex[i] = ex;
-        if (o.ey) o.ey[i] = ey;
-        if (o.ez) o.ez[i] = ez;
+        if (o.ex) o.ex[i] = (double)ex;
+        if (o.ey) o.ey[i] = (double)ey;
+        if (o.ez) o.ez[i] = (double)ez;
         if (o.acc_pm) {
             o.acc_pm[3 * (size_t)i + 0] = apx;
             o.acc_pm[3 * (size_t)i + 1] = apy;
@@ -374,16 +390,25 @@ int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
     P.spos = c->spos;
     P.cell_start = c->cell_start;
     P.perm = c->perm;
-    P.emesh = c->emesh;
     P.acc_pp_s = c->acc_pp_s;
     P.part = c->part_red;
     P.o = o;
+    const bool mixed = c->prm.precision == MDF_PREC_MIXED;
+    P.emesh = mixed ? (const void *)c->emeshf : (const void *)c->emesh;
+    const dim3 grid(c->brick.nblock_gather);
     if (c->brick.lds_gather) {
-        const size_t lds = 3 * sizeof(double) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2];
-        MDF_TRY(allow_lds(k_gather<true>, lds));
-        hipLaunchKernelGGL(k_gather<true>, dim3(c->brick.nblock_gather), dim3(kGatherThreads), lds, s, P);
+        const size_t lds = 3 * (mixed ? 4 : 8) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2];
+        if (mixed) {
+            MDF_TRY(allow_lds(k_gather<true, float>, lds));
+            hipLaunchKernelGGL((k_gather<true, float>), grid, dim3(kGatherThreads), lds, s, P);
+        } else {
+            MDF_TRY(allow_lds(k_gather<true, double>, lds));
+            hipLaunchKernelGGL((k_gather<true, double>), grid, dim3(kGatherThreads), lds, s, P);
+        }
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_gather<false, float>), grid, dim3(kGatherThreads), 0, s, P);
     } else {
-        hipLaunchKernelGGL(k_gather<false>, dim3(c->brick.nblock_gather), dim3(kGatherThreads), 0, s, P);
+        hipLaunchKernelGGL((k_gather<false, double>), grid, dim3(kGatherThreads), 0, s, P);
     }
     MDF_LAUNCHED();
     return MDF_OK;

@@ -35,23 +35,58 @@
 //   The energy is halved at the end: U_s_r = 1/2 sum_i sum_{j != i} phi.
 // * A tile larger than the LDS capacity is processed in several passes over
 //   its rows (i's force accumulates in global memory across passes).
+// * Mixed precision (k_pp<float>): each tile coordinate, taken relative to
+//   the centre of the home block in FP64, is stored as an FP32 pair hi + lo
+//   (hi = fl32(c), lo = fl32(c - hi)).  The candidate sweep reads only hi;
+//   a queued pair's displacement is (xi_hi - xj_hi) + (xi_lo - xj_lo): the
+//   hi difference of two nearby coordinates is exact (Sterbenz), so the
+//   displacement keeps FP32 relative accuracy even for pairs far closer than
+//   the hi rounding (2^-24 of 1.5 cells).  The pair terms use FP32 tables
+//   (degree kChebDegF, fitted to 1e-7 of max|g|) and FP32 arithmetic; each
+//   lane's few pair sums are widened to FP64 before the half-wave and block
+//   reductions.  Self pairs are excluded by tile slot, coincident particles
+//   (zero displacement) by the pair test.
 #include "mdf_internal.h"
 
 namespace mdf {
 
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using t = double2;
+};
+template <>
+struct Vec2<float> {
+    using t = float2;
+};
+
+template <class T>
 struct PPArgs {
+    using T2 = typename Vec2<T>::t;
     Geom g;
     const double4 *__restrict__ spos;
     const int *__restrict__ cell_start;
-    const double2 *__restrict__ cheb;  // [piece][k] = (g1 coeff, g2 coeff)
+    const T2 *__restrict__ cheb;       // [piece][k] = (g1 coeff, g2 coeff)
     double *__restrict__ acc;          // sorted (n, 3)
     double *__restrict__ part;         // per-block energy partial
     int *__restrict__ stats;
     int tile_max;
     int npiece;
-    double piece_w, inv_piece_w;
-    double rc2;                        // rc^2, the pair cutoff
-    double rc2m;                       // candidate test radius^2 (rc slightly widened)
+    T piece_w, inv_piece_w;
+    T rc2;                             // rc^2, the pair cutoff
+    T rc2m;                            // candidate test radius^2 (rc slightly widened)
+};
+
+template <class T>
+struct ChebDeg;
+template <>
+struct ChebDeg<double> {
+    static constexpr int v = kChebDeg;
+};
+template <>
+struct ChebDeg<float> {
+    static constexpr int v = kChebDegF;
 };
 
 constexpr int kGroups = 2;             // i per wave
@@ -59,14 +94,16 @@ constexpr int kGL = 64 / kGroups;      // lanes per i
 constexpr int kQCap = 2 * kGL;         // queue entries per group
 
 // Clenshaw sums of both series of one piece at t in [-1, 1].
-__device__ __forceinline__ void cheb2(const double2 *__restrict__ c, double t, double &g1, double &g2) {
-    const double t2 = t + t;
-    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+template <class T, class T2>
+__device__ __forceinline__ void cheb2(const T2 *__restrict__ c, T t, T &g1, T &g2) {
+    constexpr int D = ChebDeg<T>::v;
+    const T t2 = t + t;
+    T a1 = 0, a2 = 0, b1 = 0, b2 = 0;
 #pragma unroll
-    for (int k = kChebDeg; k >= 1; --k) {
-        const double2 ck = c[k];
-        const double na = fma(t2, a1, ck.x - a2);
-        const double nb = fma(t2, b1, ck.y - b2);
+    for (int k = D; k >= 1; --k) {
+        const T2 ck = c[k];
+        const T na = fma(t2, a1, ck.x - a2);
+        const T nb = fma(t2, b1, ck.y - b2);
         a2 = a1;
         a1 = na;
         b2 = b1;
@@ -76,19 +113,33 @@ __device__ __forceinline__ void cheb2(const double2 *__restrict__ c, double t, d
     g2 = fma(t, b1, c[0].y - b2);
 }
 
-__device__ __forceinline__ void pp_pair_cheb(const PPArgs &P, const double2 *__restrict__ tab, double dx, double dy,
-                                             double dz, double &fx, double &fy, double &fz, double &u) {
-    const double r2 = dx * dx + dy * dy + dz * dz;
+template <class T>
+__device__ __forceinline__ T rsqrt_t(T x);
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double x) {
+    return rsqrt(x);
+}
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float x) {
+    return rsqrtf(x);
+}
+
+template <class T>
+__device__ __forceinline__ void pp_pair_cheb(const PPArgs<T> &P, const typename Vec2<T>::t *__restrict__ tab, T dx,
+                                             T dy, T dz, T &fx, T &fy, T &fz, T &u) {
+    constexpr int D = ChebDeg<T>::v;
+    const T r2 = dx * dx + dy * dy + dz * dz;
     if (!(r2 < P.rc2)) return;  // r < rc (yukawa_pp.py:140) tested as r^2 < rc^2
-    const double ir = rsqrt(r2);
-    const double r = r2 * ir;
+    if (sizeof(T) == 4 && !(r2 > (T)0)) return;
+    const T ir = rsqrt_t<T>(r2);
+    const T r = r2 * ir;
     int p = (int)(r * P.inv_piece_w);
     p = p < P.npiece - 1 ? p : P.npiece - 1;
-    const double t = (r - p * P.piece_w) * (2.0 * P.inv_piece_w) - 1.0;
-    double g1, g2;
-    cheb2(tab + p * (kChebDeg + 1), t, g1, g2);
+    const T t = (r - p * P.piece_w) * ((T)2 * P.inv_piece_w) - (T)1;
+    T g1, g2;
+    cheb2<T>(tab + p * (D + 1), t, g1, g2);
     u += g1 * ir;
-    const double fr = g2 * ir * ir * ir;
+    const T fr = g2 * ir * ir * ir;
     fx += fr * dx;
     fy += fr * dy;
     fz += fr * dz;
@@ -109,14 +160,21 @@ __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, doubl
 
 __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-__global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
-    extern __shared__ double lds[];
+template <class R>
+__global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
+    using R2 = typename Vec2<R>::t;
+    constexpr int D = ChebDeg<R>::v;
+    constexpr bool kRel = sizeof(R) == 4;  // tile relative to the home block centre
+    extern __shared__ __align__(16) unsigned char ldsraw[];
     const int T = P.tile_max;
-    double *tx = lds;
-    double *ty = tx + T;
-    double *tz = ty + T;
-    double2 *tab = (double2 *)(tz + T);                    // npiece * (kChebDeg + 1)
-    int *cum = (int *)(tab + P.npiece * (kChebDeg + 1));   // 217 prefix offsets of the tile sub-cells
+    R *tx = (R *)ldsraw;
+    R *ty = tx + T;
+    R *tz = ty + T;
+    R *lx = tz + T;                                        // lo parts (mixed only)
+    R *ly = lx + (kRel ? T : 0);
+    R *lz = ly + (kRel ? T : 0);
+    R2 *tab = (R2 *)(lz + (kRel ? T : 0));                 // npiece * (D + 1)
+    int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // kPPWaves * kGroups * kQCap
     __shared__ int tcell[216];
     __shared__ int2 rng[8][25];  // per home sub-cell: LDS range of each of its 25 stencil rows
@@ -130,7 +188,11 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
     const int by = (bid / g.nblk[0]) % g.nblk[1];
     const int bz = bid / (g.nblk[0] * g.nblk[1]);
 
-    for (int t = tid; t < P.npiece * (kChebDeg + 1); t += blockDim.x) tab[t] = P.cheb[t];
+    for (int t = tid; t < P.npiece * (D + 1); t += blockDim.x) tab[t] = P.cheb[t];
+    // origin of the tile coordinates: 0, or the home block centre (mixed)
+    const double ox = kRel ? (2 * bx + 1) * g.w[0] : 0.0;
+    const double oy = kRel ? (2 * by + 1) * g.w[1] : 0.0;
+    const double oz = kRel ? (2 * bz + 1) * g.w[2] : 0.0;
     // 1. counts of the 216 tile sub-cells, tile order x fastest
     if (tid < 216) {
         const int txs = tid % 6, tys = (tid / 6) % 6, tzs = tid / 36;
@@ -181,7 +243,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
     }
     const int n_home = hoff[8];
     int *q = queue + (wave * kGroups + grp) * kQCap;
-    const double rc2m = P.rc2m;
+    const R rc2m = P.rc2m;
     double u_acc = 0.0;
 
     int r0 = 0, pass = 0;
@@ -221,9 +283,20 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
                 const int dst = cum[6 * r + txs] - base;
                 for (int t = lane; t < ge - gs; t += 64) {
                     const double4 p = P.spos[gs + t];
-                    tx[dst + t] = p.x + sx;
-                    ty[dst + t] = p.y + sy;
-                    tz[dst + t] = p.z + sz;
+                    if (kRel) {
+                        const double cx = (p.x + sx) - ox, cy = (p.y + sy) - oy, cz = (p.z + sz) - oz;
+                        const R hx = (R)cx, hy = (R)cy, hz = (R)cz;
+                        tx[dst + t] = hx;
+                        ty[dst + t] = hy;
+                        tz[dst + t] = hz;
+                        lx[dst + t] = (R)(cx - (double)hx);
+                        ly[dst + t] = (R)(cy - (double)hy);
+                        lz[dst + t] = (R)(cz - (double)hz);
+                    } else {
+                        tx[dst + t] = (R)(p.x + sx);
+                        ty[dst + t] = (R)(p.y + sy);
+                        tz[dst + t] = (R)(p.z + sz);
+                    }
                 }
                 txs = txe;
             }
@@ -256,8 +329,25 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
             const int hidx = (2 + hx) + 6 * ((2 + hy) + 6 * (2 + hz));
             const int gi = P.cell_start[tcell[hidx]] + (kk - hb);
             const double4 pi = P.spos[gi];
-            const double xi = pi.x, yi = pi.y, zi = pi.z;
-            double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
+            // same rounding as the staging of the home rows (image shift 0)
+            const double cxi = (pi.x + 0.0) - ox, cyi = (pi.y + 0.0) - oy, czi = (pi.z + 0.0) - oz;
+            const R xi = (R)cxi, yi = (R)cyi, zi = (R)czi;
+            const R xil = kRel ? (R)(cxi - (double)xi) : (R)0;
+            const R yil = kRel ? (R)(cyi - (double)yi) : (R)0;
+            const R zil = kRel ? (R)(czi - (double)zi) : (R)0;
+            // tile slot of i in this pass (its row may be outside the pass)
+            const int irow = (2 + hy) + 6 * (2 + hz);
+            const int islot = (irow >= r0 && irow < r1) ? cum[hidx] - base + (kk - hb) : -1;
+            R fx = 0, fy = 0, fz = 0, u = 0;
+            auto pair_at = [&](int jj) {
+                R dx = xi - tx[jj], dy = yi - ty[jj], dz = zi - tz[jj];
+                if (kRel) {
+                    dx += xil - lx[jj];
+                    dy += yil - ly[jj];
+                    dz += zil - lz[jj];
+                }
+                pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
+            };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
             for (int rr = 0; rr < 25; ++rr) {
                 int2 lr = rng[h][rr];
@@ -269,9 +359,9 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
                     const int j = lo + it * kGL + gl;
                     bool hit = false;
                     if (j < hi) {
-                        const double ddx = xi - tx[j], ddy = yi - ty[j], ddz = zi - tz[j];
-                        const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        hit = (r2 < rc2m) && (r2 > 0.0);
+                        const R ddx = xi - tx[j], ddy = yi - ty[j], ddz = zi - tz[j];
+                        const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        hit = kRel ? ((r2 < rc2m) && (j != islot)) : ((r2 < rc2m) && (r2 > (R)0));
                     }
                     const unsigned long long m = __ballot(hit);
                     const unsigned mh = grp ? (unsigned)(m >> 32) : (unsigned)m;
@@ -283,7 +373,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
                         const int cnt = min(grp ? qn1 : qn0, kGL);
                         if (gl < cnt) {
                             const int jj = q[gl];
-                            pp_pair_cheb(P, tab, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
+                            pair_at(jj);
                         }
                         const int rest = (grp ? qn1 : qn0) - cnt;
                         const int moved = gl < rest ? q[kGL + gl] : 0;
@@ -295,27 +385,28 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
             }
             if (gl < (grp ? qn1 : qn0)) {
                 const int jj = q[gl];
-                pp_pair_cheb(P, tab, xi - tx[jj], yi - ty[jj], zi - tz[jj], fx, fy, fz, u);
+                pair_at(jj);
             }
+            double sfx = fx, sfy = fy, sfz = fz, su = u;
 #pragma unroll
             for (int o = kGL / 2; o > 0; o >>= 1) {
-                fx += __shfl_xor(fx, o, 64);
-                fy += __shfl_xor(fy, o, 64);
-                fz += __shfl_xor(fz, o, 64);
-                u += __shfl_xor(u, o, 64);
+                sfx += __shfl_xor(sfx, o, 64);
+                sfy += __shfl_xor(sfy, o, 64);
+                sfz += __shfl_xor(sfz, o, 64);
+                su += __shfl_xor(su, o, 64);
             }
             if (gl == 0 && have_i) {
                 double *a = P.acc + 3 * (size_t)gi;
                 if (pass == 0) {
-                    a[0] = fx;
-                    a[1] = fy;
-                    a[2] = fz;
+                    a[0] = sfx;
+                    a[1] = sfy;
+                    a[2] = sfz;
                 } else {
-                    a[0] += fx;
-                    a[1] += fy;
-                    a[2] += fz;
+                    a[0] += sfx;
+                    a[1] += sfy;
+                    a[2] += sfz;
                 }
-                u_acc += u;
+                u_acc += su;
             }
         }
         __syncthreads();
@@ -332,32 +423,40 @@ __global__ __launch_bounds__(kPPThreads) void k_pp_fp64(PPArgs P) {
     }
 }
 
-size_t pp_lds_bytes(int tile_max, int npiece) {
-    return 24 * (size_t)tile_max + 16 * (size_t)npiece * (kChebDeg + 1) +
-           4 * (size_t)(224 + kPPWaves * kGroups * kQCap);
+size_t pp_lds_bytes(int tile_max, int npiece, bool mixed) {
+    // tile: (x, y, z) in FP64, or (x, y, z) hi and lo in FP32: 24 bytes a particle
+    const size_t es = mixed ? 4 : 8;
+    const size_t ncoef = (size_t)npiece * ((mixed ? kChebDegF : kChebDeg) + 1);
+    return 24 * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + kPPWaves * kGroups * kQCap);
 }
 
-int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s) {
-    (void)n;
-    PPArgs P;
+template <class R>
+static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_t s) {
+    PPArgs<R> P;
     P.g = c->g;
     P.spos = c->spos;
     P.cell_start = c->cell_start;
-    P.cheb = c->cheb;
+    P.cheb = cheb;
     P.acc = c->acc_pp_s;
     P.part = c->part_pp;
     P.stats = c->stats;
     P.tile_max = c->tile_max;
     P.npiece = c->npiece;
-    P.piece_w = c->g.rc / c->npiece;
-    P.inv_piece_w = c->npiece / c->g.rc;
+    P.piece_w = (R)(c->g.rc / c->npiece);
+    P.inv_piece_w = (R)(c->npiece / c->g.rc);
     const double rcm = c->g.rc * (1.0 + 1e-12);
-    P.rc2 = c->g.rc * c->g.rc;
-    P.rc2m = rcm * rcm;
-    MDF_TRY(allow_lds(k_pp_fp64, c->pp_lds));
-    hipLaunchKernelGGL(k_pp_fp64, dim3(c->nblock), dim3(kPPThreads), c->pp_lds, s, P);
+    P.rc2 = (R)(c->g.rc * c->g.rc);
+    P.rc2m = sizeof(R) == 4 ? P.rc2 : (R)(rcm * rcm);
+    MDF_TRY(allow_lds(k_pp<R>, c->pp_lds));
+    hipLaunchKernelGGL(k_pp<R>, dim3(c->nblock), dim3(kPPThreads), c->pp_lds, s, P);
     MDF_LAUNCHED();
     return MDF_OK;
+}
+
+int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s) {
+    (void)n;
+    if (c->prm.precision == MDF_PREC_MIXED) return launch_pp<float>(c, c->chebf, s);
+    return launch_pp<double>(c, c->cheb, s);
 }
 
 }  // namespace mdf
