@@ -138,6 +138,31 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout);
  * slower), accumulated since create.  host pointer, 2 ints. */
 int mdf_p3m_stats(mdf_p3m_ctx *ctx, int *out2);
 
+/* Replica ensembles (no single reference counterpart: the reference runs
+ * each seeded replica of yukawa_MD_p3m_ord6.py:195-219 as its own process
+ * and averages S(q,w) afterwards, s_of_q_w_avg.py).  A batch owns one
+ * context per replica (params[r]: its own box, mesh, kappa, alpha, rc,
+ * precision) and n_streams worker streams (0 = 8).
+ * mdf_batch_context: the replica's context, e.g. to set or build its Green
+ *   function (every replica needs one before mdf_batch_run).
+ * mdf_batch_run: nsteps NVE steps of every replica (the mdf_md_run step,
+ *   n(q) on the grid q_m = m*2*pi/L_r, m = 1..nq, nq <= 64).  Arrays are
+ *   host arrays of nrep device pointers: pos/vel/acc (n[r],3) updated in
+ *   place; energies[r] (nsteps*4: E, K, U, T) may be NULL (or the whole
+ *   array NULL); nqt[r] (nsteps*nq*3 complex) when nq > 0.  With
+ *   graph_steps = K > 0 the steps of all replicas are captured into one
+ *   hipGraph of K steps (recaptured when any argument changes) and replayed
+ *   nsteps/K times, replicas on different worker streams overlapping; the
+ *   remaining steps run eagerly on `stream`.  Calls on one batch must not
+ *   overlap. */
+typedef struct mdf_batch mdf_batch;
+int mdf_batch_create(int nrep, const mdf_p3m_params *params, int n_streams, mdf_batch **out);
+int mdf_batch_destroy(mdf_batch *batch);
+int mdf_batch_context(mdf_batch *batch, int r, mdf_p3m_ctx **out);
+int mdf_batch_run(mdf_batch *batch, double *const *pos, double *const *vel, double *const *acc,
+                  const int *n, int nsteps, double dt, double af, double uf, double kf, int nq,
+                  double *const *energies, double *const *nqt, int graph_steps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,120 @@
+"""Autotuner: P3M splitting parameters for a force-error budget.
+
+For one system (N ions in a cubic box of side L at screening kappa, with its
+own configuration on the GPU) the autotuner chooses the mesh M, the Ewald
+parameter alpha and the PP cutoff rc with the lowest modelled step cost whose
+measured RMS force error stays within the budget.  The error of a candidate
+is measured, not estimated: the candidate's P3M force on the system's own
+positions is compared with a tight FP64 P3M force (alpha * rc >= 4.4, fine
+mesh) computed once per system.  The error measure is the one the force
+budget is stated in: sqrt(mean_i |F_i - F_i^ref|^2) / mean_i |F_i^ref|.
+
+Cost model (seconds per step), fitted to the N = 1e6, 128^3 stage times of
+the FP32-mixed engine on MI355X:
+    t = a_pp * N * rc^3 + a_pm * N + a_fft * M^3 * log2(M^3) / 21 + t_0
+with a_pp = 1.1e-11 (PP at unit density: rc^3 in-cutoff pairs per
+particle), a_pm = 3.8e-10 (assignment + gather), a_fft = 6.1e-11 (FFT
+solve per mesh node at M = 128, scaled by log2(M^3)/21), t_0 = 2e-5 (launch
+overheads per step of a replica).
+
+The PP cell list needs L/rc >= 3, so rc <= L/3.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import torch
+
+A_PP = 1.1e-11
+A_PM = 3.8e-10
+A_FFT = 6.1e-11
+T_0 = 2e-5
+
+
+def model_cost(N, M, rc):
+    """Modelled seconds per NVE step of one system."""
+    M3 = M ** 3
+    return A_PP * N * rc ** 3 + A_PM * N + A_FFT * M3 * math.log2(M3) / 21.0 + T_0
+
+
+def force_error(acc, ref):
+    """RMS force error relative to the mean reference force magnitude."""
+    d = acc - ref
+    rms = torch.sqrt(torch.mean(torch.sum(d * d, 1)))
+    mean_f = torch.mean(torch.sqrt(torch.sum(ref * ref, 1)))
+    return float(rms / mean_f)
+
+
+@dataclass
+class Choice:
+    mesh: int
+    alpha: float
+    rc: float
+    error: float
+    cost: float
+    tried: list = field(default_factory=list)
+
+
+def reference_force(P3M, pos, L, kappa, device=None):
+    """Tight FP64 P3M force of a configuration (unit charges, af = 1)."""
+    rc = min(9.0, L / 3.0 * (1.0 - 1e-9))
+    alpha = 4.6 / rc
+    mesh = 128 if L > 24.0 else 64
+    e = P3M(pos.shape[0], mesh, L, kappa, alpha, rc, device=device)
+    e.build_green()
+    acc, _ = e.force(pos, None, af=1.0)
+    acc = acc.clone()
+    e.close()
+    return acc
+
+
+def tune(pos, L, kappa, budget=1e-5, meshes=(32, 64), rcs=None, alphas=None, precision="mixed", margin=0.8,
+         ref=None, device=None, fallback_meshes=(128,)):
+    """Cheapest (mesh, alpha, rc) with measured error <= margin * budget.
+
+    pos: (N, 3) float64 GPU tensor of the system's positions.  The meshes in
+    ``fallback_meshes`` are tried only when none of ``meshes`` meets the
+    budget (weak screening in a large box).  Returns a Choice; raises if no
+    candidate meets the budget.
+    """
+    try:
+        return _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device)
+    except RuntimeError:
+        if not fallback_meshes:
+            raise
+        return _tune(pos, L, kappa, budget, fallback_meshes, rcs, alphas, precision, margin, ref, device)
+
+
+def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device):
+    from .engine import P3M
+
+    N = pos.shape[0]
+    if ref is None:
+        ref = reference_force(P3M, pos, L, kappa, device=device)
+    rc_max = L / 3.0 * (1.0 - 1e-9)
+    if rcs is None:
+        rcs = [r for r in (3.0 + 0.5 * k for k in range(19)) if r <= rc_max]
+        if not rcs:
+            rcs = [rc_max]
+    if alphas is None:
+        alphas = (0.25, 0.3, 0.35, 0.42, 0.5, 0.6, 0.72, 0.86, 1.03)
+    cands = sorted(((model_cost(N, M, rc), M, rc) for M in meshes for rc in rcs))
+    tried = []
+    for cost, M, rc in cands:
+        best = None
+        for a in alphas:
+            # PP truncation alone is about exp(-(alpha rc)^2): skip hopeless pairs
+            if a * rc < 2.6 or a * rc > 5.0:
+                continue
+            e = P3M(N, M, L, kappa, a, rc, precision=precision, device=device)
+            e.build_green()
+            acc, _ = e.force(pos, None, af=1.0)
+            err = force_error(acc, ref)
+            e.close()
+            tried.append((M, a, rc, err))
+            if best is None or err < best[1]:
+                best = (a, err)
+        if best is not None and best[1] <= margin * budget:
+            return Choice(M, best[0], rc, best[1], cost, tried)
+    raise RuntimeError(f"no (mesh, alpha, rc) meets the force budget {budget} (N={N}, L={L:.3f}, kappa={kappa})")

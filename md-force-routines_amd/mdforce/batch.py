@@ -1,0 +1,123 @@
+"""Replica ensembles on one GPU (mdf_batch_* of the C ABI).
+
+Many independent seeded systems - the runs whose n(q, t) the reference
+averages into S(q, w) (s_of_q_w_avg.py) - advance together: each replica has
+its own context (box, mesh, kappa, alpha, rc), and the NVE steps of all
+replicas are captured into one hipGraph of ``graph_steps`` steps that is
+replayed, with replicas overlapping on worker streams.  n(q, t) on the grid
+q_m = m * 2 pi / L_r (m = 1..nq, along x, y and z) and the per-step (E, K,
+U, T) are written to device arrays for every step.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import torch
+
+from ._lib import P3MParams, check, lib
+from .engine import _PREC, P3M, _need
+
+
+class _Replica(P3M):
+    """A P3M view of a context owned by a batch (not destroyed with the view)."""
+
+    def __init__(self, ctx, n_max, mesh, box, kappa, alpha, rc, precision, device):
+        self.device = device
+        self.n_max = n_max
+        self.mesh = (mesh, mesh, mesh)
+        self.box = (box, box, box)
+        self.kappa, self.alpha, self.rc = kappa, alpha, rc
+        self.precision = precision
+        self._lib = lib()
+        self._ctx = ctx
+        self.energies = torch.zeros(4, dtype=torch.float64, device=device)
+        self._scratch = torch.zeros(4, dtype=torch.float64, device=device)
+        self.has_green = False
+
+    def close(self):
+        self._ctx = ct.c_void_p()
+
+
+class ReplicaBatch:
+    """systems: list of dicts with keys n, L, kappa, mesh, alpha, rc (and
+    optionally precision, default "mixed")."""
+
+    def __init__(self, systems, n_streams=8, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("mdforce needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.systems = [dict(s) for s in systems]
+        R = len(self.systems)
+        arr = (P3MParams * R)()
+        for r, s in enumerate(self.systems):
+            p = arr[r]
+            p.n_max = int(s["n"])
+            for d in range(3):
+                p.mesh[d] = int(s["mesh"])
+                p.box[d] = float(s["L"])
+            p.kappa, p.alpha, p.rc = float(s["kappa"]), float(s["alpha"]), float(s["rc"])
+            p.order = 6
+            p.precision = _PREC[s.get("precision", "mixed")]
+        self._lib = lib()
+        self._b = ct.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self._lib.mdf_batch_create(R, arr, int(n_streams), ct.byref(self._b)), "mdf_batch_create")
+        self.replicas = []
+        for r, s in enumerate(self.systems):
+            c = ct.c_void_p()
+            check(self._lib.mdf_batch_context(self._b, r, ct.byref(c)), "mdf_batch_context")
+            rep = _Replica(c, int(s["n"]), int(s["mesh"]), float(s["L"]), float(s["kappa"]), float(s["alpha"]),
+                           float(s["rc"]), s.get("precision", "mixed"), self.device)
+            rep.build_green()
+            self.replicas.append(rep)
+        self._keep = None
+
+    def __len__(self):
+        return len(self.systems)
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            for rep in self.replicas:
+                rep.close()
+            self._lib.mdf_batch_destroy(self._b)
+            self._b = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, pos, vel, acc, nsteps, dt, nq=20, energies=None, nqt=None, graph_steps=50, af=1.0 / 3.0,
+            uf=1.0, kf=1.5):
+        """nsteps NVE steps of every replica in place.
+
+        pos/vel/acc: lists of (n_r, 3) float64 GPU tensors (acc holds the
+        force of pos on entry).  Returns (energies, nqt): lists of (nsteps, 4)
+        and (nsteps, nq, 3) complex tensors (allocated if not given).
+        """
+        R = len(self.systems)
+        if not (len(pos) == len(vel) == len(acc) == R):
+            raise ValueError("pos, vel and acc need one tensor per replica")
+        ns = []
+        for r in range(R):
+            n = pos[r].shape[0]
+            for t, name in ((pos[r], "pos"), (vel[r], "vel"), (acc[r], "acc")):
+                _need(t, f"{name}[{r}]", (n, 3), self.device)
+            ns.append(n)
+        if energies is None:
+            energies = [torch.empty((nsteps, 4), dtype=torch.float64, device=self.device) for _ in range(R)]
+        if nq and nqt is None:
+            nqt = [torch.empty((nsteps, nq, 3), dtype=torch.complex128, device=self.device) for _ in range(R)]
+        P = ct.c_void_p * R
+        pp = P(*[t.data_ptr() for t in pos])
+        vp = P(*[t.data_ptr() for t in vel])
+        ap = P(*[t.data_ptr() for t in acc])
+        ep = P(*[t.data_ptr() for t in energies])
+        qp = P(*[t.data_ptr() for t in nqt]) if nq else None
+        narr = (ct.c_int * R)(*ns)
+        stream = ct.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        check(self._lib.mdf_batch_run(self._b, pp, vp, ap, narr, int(nsteps), float(dt), float(af), float(uf),
+                                      float(kf), int(nq), ep, qp, int(graph_steps), stream), "mdf_batch_run")
+        self._keep = (pos, vel, acc, energies, nqt)
+        return energies, (nqt if nq else None)
