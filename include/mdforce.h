@@ -163,6 +163,53 @@ int mdf_batch_run(mdf_batch *batch, double *const *pos, double *const *vel, doub
                   const int *n, int nsteps, double dt, double af, double uf, double kf, int nq,
                   double *const *energies, double *const *nqt, int graph_steps, void *stream);
 
+/* Learned short-range correction (no reference counterpart: a build-side
+ * addition that lets the PP cutoff and the mesh shrink).  A pair network
+ * (16 features -> 3 x 64 SiLU -> 1, BF16 weights and activations, FP32
+ * accumulation on MFMA) predicts a radial force g_ij for every ordered pair
+ * with r_ij < rc of the context; particle i gains sum_j g_ij (x_i - x_j)/r_ij.
+ * Features: 12 Gaussians exp(-((r - mu0 - k dmu) inv_sigma)^2), the pair's
+ * local density (mean of the two PP-cell counts over their mean, minus 1),
+ * and the run constants f_kappa, f_alpha, f_rc.  g = out_scale * network.
+ * Parameters: mdf_mlp_nparam() FP32 values, packed W1[16][64], b1[64],
+ * W2[64][64], b2, W3[64][64], b3, w4[64], b4 (W[k][n]: input k, unit n). */
+typedef struct {
+    float mu0, dmu, inv_sigma;   /* radial basis */
+    float f_kappa, f_alpha, f_rc; /* constant features of the setting */
+    float out_scale;             /* output scale (force units) */
+} mdf_mlp_meta;
+typedef struct mdf_mlp mdf_mlp;
+int mdf_mlp_nparam(void);
+/* n_max particles, max_pairs ordered pairs (the pair list capacity). */
+int mdf_mlp_create(int n_max, int max_pairs, const mdf_mlp_meta *meta, mdf_mlp **out);
+int mdf_mlp_destroy(mdf_mlp *mlp);
+int mdf_mlp_set_meta(mdf_mlp *mlp, const mdf_mlp_meta *meta);
+/* Copy parameters in (device FP32, may be NULL) and/or out (device, may be
+ * NULL); reset_adam != 0 clears the optimiser state. */
+int mdf_mlp_params(mdf_mlp *mlp, const float *in, float *out, int reset_adam, void *stream);
+/* Bins pos with ctx, builds the pair list (r < ctx rc) and runs the network.
+ * Optional device outputs: pairs_out (1 + 6*npairs 32-bit words: the count,
+ * then per pair i, j (sorted indices), r, u_x, u_y, u_z as float bits),
+ * g_out (npairs floats), rho_out (n floats, sorted order), perm_out (n ints:
+ * sorted -> caller index).  Synchronises; returns npairs (>= 0) or an error. */
+int mdf_mlp_pairs(mdf_mlp *mlp, mdf_p3m_ctx *ctx, const double *pos, int n, int *pairs_out,
+                  float *g_out, float *rho_out, int *perm_out, void *stream);
+/* f_out (n,3) <- the network's correction force for pos (caller order). */
+int mdf_mlp_correction(mdf_mlp *mlp, mdf_p3m_ctx *ctx, const double *pos, int n, double *f_out,
+                       void *stream);
+/* Loss (1/n) sum_i |F_i - target_i|^2 of the correction F against target
+ * (n,3, caller order) and its gradient (device FP32, nparam) from the BF16
+ * MFMA backward pass.  build != 0 bins pos and rebuilds the pair list first. */
+int mdf_mlp_grad(mdf_mlp *mlp, mdf_p3m_ctx *ctx, const double *pos, const double *target, int n,
+                 float *grad_out, double *loss_out, int build, void *stream);
+/* One Adam step on the last gradient. */
+int mdf_mlp_adam(mdf_mlp *mlp, float lr, float beta1, float beta2, float eps, void *stream);
+/* host out2: [pair count of the last list over n particles, overflow flag] */
+int mdf_mlp_stats(mdf_mlp *mlp, int n, int *out2);
+/* Attach (or detach with NULL) a network to a context: mdf_p3m_force and
+ * mdf_md_run then add its correction to the PP force of every evaluation. */
+int mdf_p3m_attach_mlp(mdf_p3m_ctx *ctx, mdf_mlp *mlp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -379,6 +379,7 @@ int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, d
     MDF_TRY(bin_particles(ctx, pos, Z, n, s));
     prof_mark(ctx, kStagePP, s);
     MDF_TRY(pp_forces(ctx, n, s));
+    if (ctx->mlp) MDF_TRY(mlp_correct(ctx->mlp, ctx, n, s));
     prof_mark(ctx, kStageAssign, s);
     MDF_TRY(pm_assign(ctx, n, rho, s));
     prof_mark(ctx, kStageFFT, s);

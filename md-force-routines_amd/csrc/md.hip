@@ -382,6 +382,7 @@ int md_step(mdf_p3m_ctx *c, double *pos, double *vel, double *acc, const double 
     MDF_TRY(bin_particles(c, pos, Z, n, s));
     prof_mark(c, kStagePP, s);
     MDF_TRY(pp_forces(c, n, s));
+    if (c->mlp) MDF_TRY(mlp_correct(c->mlp, c, n, s));
     prof_mark(c, kStageAssign, s);
     MDF_TRY(pm_assign(c, n, nullptr, s));
     prof_mark(c, kStageFFT, s);

@@ -126,6 +126,7 @@ struct mdf_p3m_ctx {
     double *scal;       // device scalars, see kScal*
     int *stats;         // device Stats
     struct Prof *prof;  // optional stage timing (null when off)
+    mdf_mlp *mlp;       // attached learned PP correction (null = none)
 };
 
 namespace mdf {
@@ -176,6 +177,8 @@ int build_pair_table(double kappa, double alpha, double rc, int deg, double tol,
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed);
 // FP32 copy of the FP64 half-spectrum Green function (mixed mode).
 int green_to_float(mdf_p3m_ctx *c, hipStream_t s);
+// Learned correction of the attached network, added to acc_pp_s.
+int mlp_correct(mdf_mlp *m, mdf_p3m_ctx *c, int n, hipStream_t s);
 
 // Fill helpers (kernels, so they can live inside captured graphs).
 int fill_zero(double *p, size_t n, hipStream_t s);

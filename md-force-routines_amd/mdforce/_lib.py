@@ -33,6 +33,14 @@ class P3MParams(ct.Structure):
     ]
 
 
+class MLPMeta(ct.Structure):
+    _fields_ = [
+        ("mu0", ct.c_float), ("dmu", ct.c_float), ("inv_sigma", ct.c_float),
+        ("f_kappa", ct.c_float), ("f_alpha", ct.c_float), ("f_rc", ct.c_float),
+        ("out_scale", ct.c_float),
+    ]
+
+
 _vp = ct.c_void_p
 _i = ct.c_int
 _d = ct.c_double
@@ -57,6 +65,17 @@ SIGNATURES = {
     "mdf_batch_destroy": [_vp],
     "mdf_batch_context": [_vp, _i, ct.POINTER(_vp)],
     "mdf_batch_run": [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _vp, _vp, _i, _vp],
+    "mdf_mlp_nparam": [],
+    "mdf_mlp_create": [_i, _i, ct.POINTER(MLPMeta), ct.POINTER(_vp)],
+    "mdf_mlp_destroy": [_vp],
+    "mdf_mlp_set_meta": [_vp, ct.POINTER(MLPMeta)],
+    "mdf_mlp_params": [_vp, _vp, _vp, _i, _vp],
+    "mdf_mlp_pairs": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "mdf_mlp_correction": [_vp, _vp, _vp, _i, _vp, _vp],
+    "mdf_mlp_grad": [_vp, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp],
+    "mdf_mlp_adam": [_vp, ct.c_float, ct.c_float, ct.c_float, ct.c_float, _vp],
+    "mdf_mlp_stats": [_vp, _i, ct.POINTER(ct.c_int)],
+    "mdf_p3m_attach_mlp": [_vp, _vp],
 }
 
 _lib = None

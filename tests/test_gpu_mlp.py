@@ -1,0 +1,121 @@
+"""BF16-MFMA pair network on the GPU against the NumPy oracle (oracle/mlp_oracle.py).
+
+On the GPU's own pair list (same r, u, density features), the oracle
+restates forward, correction force, loss and gradient with the same BF16
+rounding points; the GPU accumulates in FP32 where the oracle sums in FP64,
+and an FP32 sum can move a pre-activation across a BF16 rounding boundary,
+so the bars are relative-RMS: outputs and forces <= 2e-3, loss <= 1e-4 and
+gradient (norm-wise) <= 1e-2.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def setup():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test run without a GPU")
+    from mdforce import mlp
+    from mdforce.engine import P3M
+    N = 3000
+    L = (4 * math.pi * N / 3) ** (1 / 3)
+    kappa, alpha, rc = 2.0, 0.86, 3.5
+    e = P3M(N, 32, L, kappa, alpha, rc)
+    e.build_green()
+    meta = mlp.default_meta(kappa, alpha, rc, out_scale=0.5)
+    net = mlp.PairMLP(N, 400000, meta)
+    p = mlp.init_params(1, out_std=1.0)
+    net.set_params(p)
+    rs = np.random.RandomState(11)
+    pos = torch.from_numpy(rs.random_sample((N, 3)) * L).cuda()
+    return dict(mlp=mlp, e=e, net=net, p=p.astype(np.float64), pos=pos, N=N, meta=meta, rs=rs)
+
+
+def _oracle_inputs(d, meta):
+    pi = d["i"].cpu().numpy()
+    pj = d["j"].cpu().numpy()
+    r = d["r"].double().cpu().numpy()
+    u = d["u"].double().cpu().numpy()
+    rho = d["rho"].double().cpu().numpy()
+    x = M.features(r, M.pair_density(rho, pi, pj), meta)
+    return pi, pj, r, u, x
+
+
+def test_forward_and_correction_match_oracle(setup):
+    S = setup
+    d = S["net"].pairs(S["e"], S["pos"])
+    P = d["npairs"]
+    print(f"[mlp] N={S['N']}: {P} ordered pairs")
+    assert 40 * S["N"] < P < 50 * S["N"]  # about rc^3 = 42.9 per particle at unit Wigner-Seitz density
+    pi, pj, r, u, x = _oracle_inputs(d, S["meta"])
+    assert (r < 3.5).all() and (r > 0).all()
+    np.testing.assert_allclose(np.linalg.norm(u, axis=1), 1.0, atol=1e-6)
+    # every pair appears in both orders
+    key = set(zip(pi.tolist(), pj.tolist()))
+    assert all((b, a) in key for a, b in list(key)[:2000])
+    g_o, _ = M.forward(S["p"], x, S["meta"], bf16_mode=True)
+    g_gpu = d["g"].double().cpu().numpy()
+    print(f"[mlp] forward rel rms {rel(g_gpu, g_o):.2e}")
+    assert rel(g_gpu, g_o) <= 2e-3
+    F = S["net"].correction(S["e"], S["pos"]).cpu().numpy()
+    perm = d["perm"].cpu().numpy()
+    Fs = M.correction(g_o, pi, u, S["N"])
+    Fo = np.empty_like(Fs)
+    Fo[perm] = Fs
+    print(f"[mlp] correction rel rms {rel(F, Fo):.2e}")
+    assert rel(F, Fo) <= 2e-3
+
+
+def test_gradient_matches_oracle(setup):
+    S = setup
+    N = S["N"]
+    target = S["rs"].randn(N, 3) * 0.3
+    t_dev = torch.from_numpy(target).cuda()
+    d = S["net"].pairs(S["e"], S["pos"])
+    g, loss = S["net"].grad(S["e"], S["pos"], t_dev)
+    torch.cuda.synchronize()
+    g = g.double().cpu().numpy()
+    pi, pj, r, u, x = _oracle_inputs(d, S["meta"])
+    perm = d["perm"].cpu().numpy()
+    L_o, G_o, _ = M.loss_and_grad(S["p"], x, pi, u, target[perm], S["meta"], bf16_mode=True)
+    print(f"[mlp] loss gpu {float(loss[0]):.6e} oracle {L_o:.6e}; grad rel {rel(g, G_o):.2e}")
+    assert abs(float(loss[0]) - L_o) <= 1e-4 * L_o
+    assert rel(g, G_o) <= 1e-2
+    for name, (o, s) in M.OFF.items():
+        assert rel(g[o:o + s], G_o[o:o + s]) <= 3e-2, name
+
+
+def test_training_fits_a_teacher(setup):
+    """Adam on GPU gradients recovers the correction of a teacher network."""
+    S = setup
+    mlp = S["mlp"]
+    N = S["N"]
+    teacher = mlp.PairMLP(N, 400000, S["meta"])
+    teacher.set_params(mlp.init_params(7, out_std=1.0))
+    target = teacher.correction(S["e"], S["pos"]).clone()
+    teacher.close()
+    S["net"].set_params(mlp.init_params(3, out_std=0.1))
+    losses = []
+    for it in range(150):
+        _, loss = S["net"].grad(S["e"], S["pos"], target, build=(it == 0))
+        S["net"].adam(3e-3)
+        losses.append(float(loss[0]))
+    torch.cuda.synchronize()
+    base = float((target * target).sum(1).mean())
+    print(f"[mlp] teacher fit: loss {losses[0]:.3e} -> {losses[-1]:.3e} (|target|^2 {base:.3e})")
+    assert losses[-1] < 0.1 * losses[0]
+    S["net"].set_params(S["p"].astype(np.float32))
