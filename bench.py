@@ -60,10 +60,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-err", action="store_true", help="skip the force-error measurement")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", choices=("n1e6", "replicas"), default="n1e6",
+    ap.add_argument("--workload", choices=("n1e6", "replicas", "mlp"), default="n1e6",
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
     ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--mlp-epochs", type=int, default=20)
     ap.add_argument("--graph-steps", type=int, default=50, help="0 = no graph (eager launches)")
     ap.add_argument("--streams", type=int, default=16, help="worker streams of the replica graph")
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both",
@@ -194,6 +195,102 @@ def run_replicas(a):
     print(json.dumps(out), flush=True)
 
 
+def run_mlp(a):
+    """Learned short-range correction.  Training (outside the timed region):
+    16 states Gamma in {10, 50, 100, 175} x kappa in {0.5, 1, 2, 3}, 10
+    equilibrated snapshots each of N = 125,000 (2e7 force residuals) between
+    tight FP64 P3M (rc 8, 128^3) and the reduced setting (rc 3.5, 64^3,
+    alpha 0.6; mesh spacing 1.26, that of N = 1e6 on 128^3).  Evaluation at
+    N = 1e6, Gamma 100, kappa 2, equilibrated: RMS force error against a tight
+    FP64 P3M (rc 9, alpha 0.5, 256^3) of the reduced setting with and without
+    the correction, the plain P3M judge's estimate and the resulting form,
+    and the NVE throughput of the corrected reduced step and of the plain
+    FP64 step."""
+    import numpy as np
+    import torch
+
+    from mdforce import mlp_train as T
+    from mdforce.mlp import PairMLP, default_meta
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    Ntr, rc_red, a_red = 125000, 3.5, 0.6
+    red = lambda k: (64, a_red, rc_red)
+    tight = lambda k: (128, 0.55, 8.0)
+    prop = lambda k: (64, 0.48, 5.5)
+    states = [(g, k) for g in (10.0, 50.0, 100.0, 175.0) for k in (0.5, 1.0, 2.0, 3.0)]
+    t0 = time.perf_counter()
+    snaps = T.make_snapshots(Ntr, states, 10, 20, 300, red=red, tight=tight, prop=prop, seed0=100, log=log)
+    t_data = time.perf_counter() - t0
+    rms = T.residual_rms(snaps)
+    engs = {k: T.Setting(Ntr, *red(k), k).eng for k in (0.5, 1.0, 2.0, 3.0)}
+    net = PairMLP(1_000_000, 50_000_000, default_meta(2.0, a_red, rc_red, out_scale=rms / 5.0), device=dev)
+    t0 = time.perf_counter()
+    hist = T.train(net, engs, snaps, a.mlp_epochs, 2e-3, lr_end=2e-4, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del snaps
+    # evaluation system: N = 1e6, Gamma 100, kappa 2, equilibrated
+    N, kappa, gamma = 1_000_000, 2.0, 100.0
+    ev = T.make_snapshots(N, [(gamma, kappa)], 1, 20, 300, red=lambda k: (128, a_red, rc_red),
+                          tight=lambda k: (256, 0.5, 9.0), prop=lambda k: (128, 0.48, 5.5), seed0=5000, log=log)[0]
+    pos = ev["pos"]
+    Sr = T.Setting(N, 128, a_red, rc_red, kappa)
+    St = T.Setting(N, 256, 0.5, 9.0, kappa)
+    Fref = T.forces(St.eng, pos)
+    St.close()
+    meanF = float(torch.sqrt((Fref * Fref).sum(1)).mean())
+    Fr = T.forces(Sr.eng, pos)
+    net.set_meta(f_kappa=kappa / 3.0, f_alpha=a_red)
+    Fc = Fr + net.correction(Sr.eng, pos)
+    e_red = float(torch.sqrt(((Fr - Fref) ** 2).sum(1).mean())) / meanF
+    e_cor = float(torch.sqrt(((Fc - Fref) ** 2).sum(1).mean())) / meanF
+    Sp = T.Setting(N, 128, 0.48, 5.5, kappa)
+    form, est = T.choose_form(net, Sr.eng, Sp.eng, pos, kappa, a_red, 1e-5)
+
+    def timed(eng, steps):
+        p = pos.clone()
+        v = torch.zeros_like(p)
+        acc, _ = eng.force(p)
+        acc = acc.clone()
+        eng.md_run(p, v, acc, 2, 0.1)
+        torch.cuda.synchronize()
+        s0 = torch.cuda.Event(enable_timing=True)
+        s1 = torch.cuda.Event(enable_timing=True)
+        s0.record()
+        eng.md_run(p, v, acc, steps, 0.1)
+        s1.record()
+        torch.cuda.synchronize()
+        return s0.elapsed_time(s1) / steps
+
+    net.attach(Sr.eng)
+    ms_learned = timed(Sr.eng, a.steps)
+    net.detach(Sr.eng)
+    ms_red = timed(Sr.eng, a.steps)
+    ms_plain = timed(Sp.eng, a.steps)
+    used = ms_learned if form == "learned" else ms_plain
+    out = {
+        "metric": METRIC + " (learned correction)",
+        "value": N / (used / 1e3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": 2, "ms_per_step": used,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f64", "data": "synthetic",
+        "config": {
+            "workload": "learned PP correction, train N=125000 x 160 snapshots, evaluate N=1e6 Gamma 100 kappa 2",
+            "reduced": {"mesh": 128, "alpha": a_red, "rc": rc_red},
+            "residuals": 160 * Ntr, "epochs": a.mlp_epochs, "loss_first_last": [hist[0], hist[-1]],
+            "data_s": t_data, "train_s": t_train,
+            "err_reduced_rel_meanF": e_red, "err_corrected_rel_meanF": e_cor, "gain": e_red / e_cor,
+            "judge_estimate": est, "budget": 1e-5, "form_used": form,
+            "ms_step_learned": ms_learned, "ms_step_reduced_no_mlp": ms_red, "ms_step_plain_fp64": ms_plain,
+            "parallelism": "single GPU",
+        },
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -202,6 +299,9 @@ def main():
         sys.exit(2)
     if a.workload == "replicas":
         run_replicas(a)
+        return
+    if a.workload == "mlp":
+        run_mlp(a)
         return
     import numpy as np
     import torch
