@@ -26,6 +26,8 @@ from dataclasses import dataclass, field
 
 import torch
 
+from ._lib import MdforceError
+
 A_PP = 1.1e-11
 A_PM = 3.8e-10
 A_FFT = 6.1e-11
@@ -60,7 +62,9 @@ def reference_force(P3M, pos, L, kappa, device=None):
     """Tight FP64 P3M force of a configuration (unit charges, af = 1)."""
     rc = min(9.0, L / 3.0 * (1.0 - 1e-9))
     alpha = 4.6 / rc
-    mesh = 128 if L > 24.0 else 64
+    mesh = 64  # smallest power of two with spacing <= 0.65 (alpha h <= 0.33 at rc = 9), at most 512
+    while mesh < 512 and L / mesh > 0.65:
+        mesh *= 2
     e = P3M(pos.shape[0], mesh, L, kappa, alpha, rc, device=device)
     e.build_green()
     acc, _ = e.force(pos, None, af=1.0)
@@ -107,7 +111,10 @@ def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, de
             # PP truncation alone is about exp(-(alpha rc)^2): skip hopeless pairs
             if a * rc < 2.6 or a * rc > 5.0:
                 continue
-            e = P3M(N, M, L, kappa, a, rc, precision=precision, device=device)
+            try:
+                e = P3M(N, M, L, kappa, a, rc, precision=precision, device=device)
+            except MdforceError:  # geometry the engine refuses (e.g. too many cells for a small rc)
+                break
             e.build_green()
             acc, _ = e.force(pos, None, af=1.0)
             err = force_error(acc, ref)
