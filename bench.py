@@ -205,7 +205,9 @@ def run_replicas(a):
         "config": {
             "workload": f"{R} replicas, N {int(Ns.min())}-{int(Ns.max())} (sum {total_n}), "
                         "Gamma {10,50,100,175}, kappa {0.5,1,2,3}, FP32-mixed, autotuned mesh/alpha/rc",
-            "graph_steps": K, "streams": a.streams, "nq": nq, "dt": a.dt,
+            "graph_steps": K, "graph_mode": ("segmented" if b.graph_mode == 0 else "per-replica streams"),
+            "streams": a.streams, "nq": nq, "dt": a.dt,
+            "sum_N_rc3": float(sum(s["n"] * s["rc"] ** 3 for s in systems)),
             "tuned_error_max": max(errs), "tuned_error_mean": float(np.mean(errs)),
             "meshes": sorted(set(s["mesh"] for s in systems)),
             "rc_range": [min(s["rc"] for s in systems), max(s["rc"] for s in systems)],

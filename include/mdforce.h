@@ -159,6 +159,13 @@ typedef struct mdf_batch mdf_batch;
 int mdf_batch_create(int nrep, const mdf_p3m_params *params, int n_streams, mdf_batch **out);
 int mdf_batch_destroy(mdf_batch *batch);
 int mdf_batch_context(mdf_batch *batch, int r, mdf_p3m_ctx **out);
+/* Graph shape: 0 (default) = segmented, each stage of the step one launch for
+ * all replicas (needs one precision in every replica and no attached
+ * learned correction, else falls back to 1); 1 = each replica's own
+ * launches on worker streams.  mdf_batch_mode returns the shape of the last captured graph
+ * (0 or 1; -1 before any capture). */
+int mdf_batch_set_mode(mdf_batch *batch, int mode);
+int mdf_batch_mode(mdf_batch *batch);
 int mdf_batch_run(mdf_batch *batch, double *const *pos, double *const *vel, double *const *acc,
                   const int *n, int nsteps, double dt, double af, double uf, double kf, int nq,
                   double *const *energies, double *const *nqt, int graph_steps, void *stream);

@@ -10,13 +10,21 @@
 //                          kick, FP64 reductions, n(q) on the q grid]
 //   <--join-- flush kernel (ring -> outputs at the launch counter), counter++
 //
-// One graph launch advances every replica by K steps; the graph's branches
-// are independent, so small replicas overlap on the chip instead of queueing
-// behind each other.  Per-step energies and n(q, t) go to ring buffers of K
+// One graph launch advances every replica by K steps.  Two graph shapes:
+//   segmented (default): every stage of the step is ONE launch for all
+//     replicas; block b belongs to the replica whose block range holds it
+//     (a prefix table), and each replica's kernel arguments come from a
+//     device table built once before capture.  ~18 launches per step
+//     whatever the number of replicas.
+//   per-replica streams (mode 1, or when a replica cannot be segmented):
+//     each replica's own launches on one of W worker streams, forked and
+//     joined inside the graph.  Per-step energies and n(q, t) go to ring buffers of K
 // steps inside the graph, and the flush kernel copies them to the caller's
 // (nsteps, ...) arrays at offset launch * K read from a device counter, so
 // the same executable graph is replayed without changing any argument.
 // Steps beyond the last whole graph run eagerly on the caller stream.
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "mdf_internal.h"
@@ -45,6 +53,19 @@ struct mdf_batch {
     };
     Flush *flush = nullptr;    // nrep entries (device)
     int *counter = nullptr;    // graph launches since the run began (device)
+    int mode = 0;              // 0: segmented when possible, 1: per-replica streams
+    int used_mode = -1;        // shape of the captured graph
+    // segmented launch tables
+    std::vector<mdf::SegStage> stages;
+    unsigned char *seg_mem = nullptr;
+    struct StageRun {
+        int stage;
+        const void *tab;
+        mdf::SegGrid S;
+        int total;
+        size_t lds;
+    };
+    std::vector<StageRun> runs;
 };
 
 namespace mdf {
@@ -93,6 +114,7 @@ static void free_batch(mdf_batch *b) {
     if (b->ring_e) (void)hipFree(b->ring_e);
     if (b->ring_q) (void)hipFree(b->ring_q);
     if (b->flush) (void)hipFree(b->flush);
+    if (b->seg_mem) (void)hipFree(b->seg_mem);
     if (b->counter) (void)hipFree(b->counter);
     delete b;
 }
@@ -110,6 +132,163 @@ static int enqueue_replica(mdf_batch *b, int r, int K, hipStream_t w) {
             double *dst = b->ring_q + (((size_t)r * K + k) * b->nq) * 6;
             MDF_TRY(density_modes_grid(c, b->cap_pos[r], b->cap_n[r], q_step_of(c), b->nq, dst, w));
         }
+    }
+    return MDF_OK;
+}
+
+// Builds every stage's argument table for all replicas and uploads them
+// (synchronously, before capture).  Replicas whose dynamic LDS differ by more
+// than 25% go to separate launches of the stage (each launch reserves its
+// largest replica's LDS for all its workgroups, so one replica with a large
+// tile would otherwise lower the occupancy of all).  Fails if a replica
+// cannot be segmented.
+static int seg_prepare(mdf_batch *b) {
+    const int R = b->nrep;
+    const int prec = b->ctx[0]->prm.precision;
+    for (int r = 0; r < R; ++r) {
+        if (b->ctx[r]->prm.precision != prec) {
+            set_error("replica batch: mixed precisions");
+            return MDF_EINVAL;
+        }
+        if (b->ctx[r]->mlp) {
+            set_error("replica batch: a replica has a learned correction attached");
+            return MDF_EINVAL;
+        }
+    }
+    seg_step_stages(prec, b->stages);
+    if (b->nq == 0) b->stages.resize(b->stages.size() - 2);  // no n(q) stages
+    const size_t nst = b->stages.size();
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    struct Group {
+        size_t stage;
+        std::vector<int> reps;
+        size_t lds = 0;
+        size_t off = 0;
+        int total = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<std::vector<unsigned char>> args(nst);
+    std::vector<std::vector<int4>> dims(nst, std::vector<int4>(R));
+    std::vector<std::vector<size_t>> lds(nst, std::vector<size_t>(R, 0));
+    for (size_t i = 0; i < nst; ++i) {
+        const SegStage &st = b->stages[i];
+        args[i].assign(st.arg_size * R, 0);
+        for (int r = 0; r < R; ++r) {
+            StepIO io;
+            io.pos = b->cap_pos[r];
+            io.vel = b->cap_vel[r];
+            io.acc = b->cap_acc[r];
+            io.n = b->cap_n[r];
+            io.dt = b->dt;
+            io.af = b->af;
+            io.uf = b->uf;
+            io.kf = b->kf;
+            io.rec0 = b->ring_e + (size_t)r * b->K * 4;
+            io.rec_stride = 4;
+            io.modes0 = b->ring_q + (size_t)r * b->K * (b->nq > 0 ? b->nq : 1) * 6;
+            io.modes_stride = b->nq * 6;
+            io.nq = b->nq;
+            const int rc = st.build(b->ctx[r], io, args[i].data() + st.arg_size * r, dims[i][r], lds[i][r]);
+            if (rc != MDF_OK) return rc;
+        }
+        // groups of similar LDS, in order of LDS
+        std::vector<int> order(R);
+        for (int r = 0; r < R; ++r) order[r] = r;
+        std::sort(order.begin(), order.end(), [&](int a, int c) { return lds[i][a] < lds[i][c]; });
+        Group g;
+        g.stage = i;
+        size_t gmin = 0;
+        for (int r : order) {
+            if (!g.reps.empty() && lds[i][r] > gmin + gmin / 4 + 1024) {
+                groups.push_back(g);
+                g = Group();
+                g.stage = i;
+            }
+            if (g.reps.empty()) gmin = lds[i][r];
+            g.reps.push_back(r);
+            if (lds[i][r] > g.lds) g.lds = lds[i][r];
+        }
+        groups.push_back(g);
+    }
+    size_t bytes = 0;
+    for (Group &g : groups) {
+        const size_t n = g.reps.size();
+        g.off = bytes;
+        bytes += align(b->stages[g.stage].arg_size * n) + align(sizeof(int) * (n + 1)) + align(sizeof(int4) * n);
+    }
+    std::vector<unsigned char> host(bytes, 0);
+    for (Group &g : groups) {
+        const SegStage &st = b->stages[g.stage];
+        const size_t n = g.reps.size();
+        unsigned char *base = host.data() + g.off;
+        int *first = (int *)(base + align(st.arg_size * n));
+        int4 *dd = (int4 *)(base + align(st.arg_size * n) + align(sizeof(int) * (n + 1)));
+        int acc = 0;
+        for (size_t q = 0; q < n; ++q) {
+            const int r = g.reps[q];
+            std::memcpy(base + st.arg_size * q, args[g.stage].data() + st.arg_size * r, st.arg_size);
+            const int4 d = dims[g.stage][r];
+            dd[q] = d;
+            first[q] = acc;
+            acc += d.x * d.y * d.z;
+        }
+        first[n] = acc;
+        g.total = acc;
+    }
+    if (b->seg_mem) (void)hipFree(b->seg_mem);
+    b->seg_mem = nullptr;
+    MDF_HIP(hipMalloc((void **)&b->seg_mem, bytes));
+    MDF_HIP(hipMemcpy(b->seg_mem, host.data(), bytes, hipMemcpyHostToDevice));
+    b->runs.clear();
+    for (const Group &g : groups) {
+        const SegStage &st = b->stages[g.stage];
+        const size_t n = g.reps.size();
+        unsigned char *base = b->seg_mem + g.off;
+        mdf_batch::StageRun u;
+        u.stage = (int)g.stage;
+        u.tab = base;
+        u.S.first = (const int *)(base + align(st.arg_size * n));
+        u.S.dims = (const int4 *)(base + align(st.arg_size * n) + align(sizeof(int) * (n + 1)));
+        u.S.nrep = (int)n;
+        u.total = g.total;
+        u.lds = g.lds;
+        b->runs.push_back(u);
+    }
+    return MDF_OK;
+}
+
+static int capture_segmented(mdf_batch *b) {
+    free_graph(b);
+    hipStream_t s = b->cap;
+    MDF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    int r = MDF_OK;
+    for (int k = 0; k < b->K && r == MDF_OK; ++k)
+        for (size_t i = 0; i < b->runs.size() && r == MDF_OK; ++i) {
+            const mdf_batch::StageRun &u = b->runs[i];
+            r = b->stages[u.stage].launch(u.tab, u.S, u.total, u.lds, k, s);
+        }
+    if (r == MDF_OK) {
+        hipLaunchKernelGGL(k_batch_flush, dim3(b->nrep), dim3(256), 0, s, b->flush, b->ring_e, b->ring_q, b->counter,
+                           b->K, b->nq);
+        hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, s, b->counter, 0);
+        if (hipGetLastError() != hipSuccess) r = MDF_EHIP;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (r != MDF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return MDF_EHIP;
+    }
+    b->graph = g;
+    const hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(ei));
+        free_graph(b);
+        return MDF_EHIP;
     }
     return MDF_OK;
 }
@@ -222,6 +401,18 @@ int mdf_batch_destroy(mdf_batch *b) {
     return MDF_OK;
 }
 
+int mdf_batch_set_mode(mdf_batch *b, int mode) {
+    if (!b || mode < 0 || mode > 1) {
+        set_error("mdf_batch_set_mode: mode must be 0 (segmented) or 1 (per-replica streams)");
+        return MDF_EINVAL;
+    }
+    if (b->mode != mode) free_graph(b);
+    b->mode = mode;
+    return MDF_OK;
+}
+
+int mdf_batch_mode(mdf_batch *b) { return b ? b->used_mode : MDF_EINVAL; }
+
 int mdf_batch_context(mdf_batch *b, int r, mdf_p3m_ctx **out) {
     if (!b || !out || r < 0 || r >= b->nrep) {
         set_error("mdf_batch_context: bad replica index");
@@ -298,7 +489,13 @@ int mdf_batch_run(mdf_batch *b, double *const *pos, double *const *vel, double *
                 tab[r].q_out = b->cap_q[r];
             }
             MDF_HIP(hipMemcpy(b->flush, tab.data(), sizeof(mdf_batch::Flush) * b->nrep, hipMemcpyHostToDevice));
-            MDF_TRY(capture(b));
+            b->used_mode = 1;
+            if (b->mode == 0 && seg_prepare(b) == MDF_OK) {
+                MDF_TRY(capture_segmented(b));
+                b->used_mode = 0;
+            } else {
+                MDF_TRY(capture(b));
+            }
         }
         hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, s, b->counter, 1);
         MDF_LAUNCHED();

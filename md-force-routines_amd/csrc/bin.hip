@@ -22,9 +22,30 @@ __device__ __forceinline__ double wrap_box(double x, double L) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-__global__ void k_bin_count(const double *__restrict__ pos, int n, Geom g, int *__restrict__ cell_count,
-                            int *__restrict__ part_cell, int *__restrict__ part_slot) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+struct BinArgs {
+    const double *pos;
+    const double *Z;
+    int n;
+    Geom g;
+    int *cell_count;
+    int *part_cell;
+    int *part_slot;
+    int ncell;
+    int *tile;         // scan tile sums (+ total at tile[ntile])
+    int ntile;
+    int *cell_start;
+    double4 *spos;
+    int *perm;
+};
+
+__device__ __forceinline__ void bin_count_body(const BinArgs &A, const Blk &b) {
+    const double *__restrict__ pos = A.pos;
+    const Geom &g = A.g;
+    const int n = A.n;
+    int *__restrict__ cell_count = A.cell_count;
+    int *__restrict__ part_cell = A.part_cell;
+    int *__restrict__ part_slot = A.part_slot;
+    const int i = b.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c[3];
 #pragma unroll
@@ -35,6 +56,13 @@ __global__ void k_bin_count(const double *__restrict__ pos, int n, Geom g, int *
     const int cell = c[0] + g.nsub[0] * (c[1] + g.nsub[1] * c[2]);
     part_cell[i] = cell;
     part_slot[i] = atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_bin_count(BinArgs A) { bin_count_body(A, blk_plain()); }
+__global__ void k_bin_count_seg(const BinArgs *__restrict__ tab, SegGrid S) {
+    Blk b;
+    const int r = seg_locate(S, b);
+    bin_count_body(tab[r], b);
 }
 
 // Exclusive scan of the cell counts in three passes over tiles of
@@ -58,21 +86,33 @@ __device__ __forceinline__ int block_incl_scan_256(int v, int *wsum) {
     return v + add;
 }
 
-__global__ __launch_bounds__(256) void k_scan_tile_sums(const int *__restrict__ cell_count, int ncell,
-                                                        int *__restrict__ tile_sum) {
-    __shared__ int wsum[4];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+__device__ __forceinline__ void scan_tile_sums_body(const BinArgs &A, const Blk &b, int *wsum) {
+    const int *__restrict__ cell_count = A.cell_count;
+    const int ncell = A.ncell;
+    const int base = b.x * kScanTile + threadIdx.x * 4;
     int s = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) s += (base + q < ncell) ? cell_count[base + q] : 0;
     const int incl = block_incl_scan_256(s, wsum);
-    if (threadIdx.x == 255) tile_sum[blockIdx.x] = incl;
+    if (threadIdx.x == 255) A.tile[b.x] = incl;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tile_sums(BinArgs A) {
+    __shared__ int wsum[4];
+    scan_tile_sums_body(A, blk_plain(), wsum);
+}
+__global__ __launch_bounds__(256) void k_scan_tile_sums_seg(const BinArgs *__restrict__ tab, SegGrid S) {
+    __shared__ int wsum[4];
+    Blk b;
+    const int r = seg_locate(S, b);
+    scan_tile_sums_body(tab[r], b, wsum);
 }
 
 // One block of 1024 threads scans up to 1024 * 8 tile sums in place (exclusive).
-__global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ tile_sum, int ntile,
-                                                     int *__restrict__ total) {
-    __shared__ int wsum[16];
+__device__ __forceinline__ void scan_tiles_body(const BinArgs &A, int *wsum) {
+    int *__restrict__ tile_sum = A.tile;
+    const int ntile = A.ntile;
+    int *__restrict__ total = A.tile + A.ntile;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     constexpr int per = 8;
     int v[per];
@@ -103,11 +143,24 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ tile_sum,
     if (t == 1023) *total = run;
 }
 
-__global__ __launch_bounds__(256) void k_scan_apply(int *__restrict__ cell_count, int ncell,
-                                                    const int *__restrict__ tile_off, const int *__restrict__ total,
-                                                    int *__restrict__ cell_start) {
-    __shared__ int wsum[4];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+__global__ __launch_bounds__(1024) void k_scan_tiles(BinArgs A) {
+    __shared__ int wsum[16];
+    scan_tiles_body(A, wsum);
+}
+__global__ __launch_bounds__(1024) void k_scan_tiles_seg(const BinArgs *__restrict__ tab, SegGrid S) {
+    __shared__ int wsum[16];
+    Blk b;
+    const int r = seg_locate(S, b);
+    scan_tiles_body(tab[r], wsum);
+}
+
+__device__ __forceinline__ void scan_apply_body(const BinArgs &A, const Blk &b, int *wsum) {
+    int *__restrict__ cell_count = A.cell_count;
+    const int ncell = A.ncell;
+    const int *__restrict__ tile_off = A.tile;
+    const int *__restrict__ total = A.tile + A.ntile;
+    int *__restrict__ cell_start = A.cell_start;
+    const int base = b.x * kScanTile + threadIdx.x * 4;
     int v[4];
     int s = 0;
 #pragma unroll
@@ -116,7 +169,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(int *__restrict__ cell_count
         s += v[q];
     }
     const int incl = block_incl_scan_256(s, wsum);
-    int run = tile_off[blockIdx.x] + incl - s;
+    int run = tile_off[b.x] + incl - s;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (base + q < ncell) {
@@ -125,14 +178,31 @@ __global__ __launch_bounds__(256) void k_scan_apply(int *__restrict__ cell_count
         }
         run += v[q];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cell_start[ncell] = *total;
+    if (b.x == 0 && threadIdx.x == 0) cell_start[ncell] = *total;
 }
 
-__global__ void k_bin_place(const double *__restrict__ pos, const double *__restrict__ Z, int n, Geom g,
-                            const int *__restrict__ cell_start, const int *__restrict__ part_cell,
-                            const int *__restrict__ part_slot, double4 *__restrict__ spos,
-                            int *__restrict__ perm) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_scan_apply(BinArgs A) {
+    __shared__ int wsum[4];
+    scan_apply_body(A, blk_plain(), wsum);
+}
+__global__ __launch_bounds__(256) void k_scan_apply_seg(const BinArgs *__restrict__ tab, SegGrid S) {
+    __shared__ int wsum[4];
+    Blk b;
+    const int r = seg_locate(S, b);
+    scan_apply_body(tab[r], b, wsum);
+}
+
+__device__ __forceinline__ void bin_place_body(const BinArgs &A, const Blk &b) {
+    const double *__restrict__ pos = A.pos;
+    const double *__restrict__ Z = A.Z;
+    const Geom &g = A.g;
+    const int n = A.n;
+    const int *__restrict__ cell_start = A.cell_start;
+    const int *__restrict__ part_cell = A.part_cell;
+    const int *__restrict__ part_slot = A.part_slot;
+    double4 *__restrict__ spos = A.spos;
+    int *__restrict__ perm = A.perm;
+    const int i = b.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = cell_start[part_cell[i]] + part_slot[i];
     const double x = wrap_box(pos[3 * i + 0], g.L[0]);
@@ -140,6 +210,13 @@ __global__ void k_bin_place(const double *__restrict__ pos, const double *__rest
     const double z = wrap_box(pos[3 * i + 2], g.L[2]);
     spos[s] = make_double4(x, y, z, Z ? Z[i] : 1.0);
     perm[s] = i;
+}
+
+__global__ void k_bin_place(BinArgs A) { bin_place_body(A, blk_plain()); }
+__global__ void k_bin_place_seg(const BinArgs *__restrict__ tab, SegGrid S) {
+    Blk b;
+    const int r = seg_locate(S, b);
+    bin_place_body(tab[r], b);
 }
 
 __global__ void k_fill_zero(double *__restrict__ p, size_t n) {
@@ -172,25 +249,82 @@ int fill_zero_i(int *p, size_t n, hipStream_t s) {
     return MDF_OK;
 }
 
+static BinArgs bin_args(mdf_p3m_ctx *c, const double *pos, const double *Z, int n) {
+    BinArgs A;
+    A.pos = pos;
+    A.Z = Z;
+    A.n = n;
+    A.g = c->g;
+    A.cell_count = c->cell_count;
+    A.part_cell = c->part_cell;
+    A.part_slot = c->part_slot;
+    A.ncell = c->ncell;
+    A.tile = c->scan_tmp;
+    A.ntile = (c->ncell + kScanTile - 1) / kScanTile;
+    A.cell_start = c->cell_start;
+    A.spos = c->spos;
+    A.perm = c->perm;
+    return A;
+}
+
 int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s) {
     if (n <= 0) return MDF_OK;
+    const BinArgs A = bin_args(c, pos, Z, n);
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, s, pos, n, c->g, c->cell_count, c->part_cell,
-                       c->part_slot);
+    hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, s, A);
     MDF_LAUNCHED();
-    const int ntile = (c->ncell + kScanTile - 1) / kScanTile;
-    int *tile = c->scan_tmp;
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntile), dim3(256), 0, s, c->cell_count, c->ncell, tile);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(A.ntile), dim3(256), 0, s, A);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile, ntile, tile + ntile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, A);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(256), 0, s, c->cell_count, c->ncell, tile, tile + ntile,
-                       c->cell_start);
+    hipLaunchKernelGGL(k_scan_apply, dim3(A.ntile), dim3(256), 0, s, A);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_bin_place, dim3(nb), dim3(256), 0, s, pos, Z, n, c->g, c->cell_start, c->part_cell,
-                       c->part_slot, c->spos, c->perm);
+    hipLaunchKernelGGL(k_bin_place, dim3(nb), dim3(256), 0, s, A);
     MDF_LAUNCHED();
     return MDF_OK;
+}
+
+// ---- segmented stages (replica batches) ----
+static int build_bin_particles(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    *(BinArgs *)args = bin_args(c, io.pos, nullptr, io.n);
+    dims = make_int4((io.n + 255) / 256 > 0 ? (io.n + 255) / 256 : 1, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int build_bin_tiles(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    const BinArgs A = bin_args(c, io.pos, nullptr, io.n);
+    *(BinArgs *)args = A;
+    dims = make_int4(A.ntile, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int build_bin_one(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    *(BinArgs *)args = bin_args(c, io.pos, nullptr, io.n);
+    dims = make_int4(1, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+#define MDF_SEG_LAUNCH(fname, kern, threads, Args)                                                        \
+    static int fname(const void *tab, const SegGrid &S, int total, size_t lds, int k, hipStream_t s) {   \
+        (void)lds;                                                                                          \
+        (void)k;                                                                                            \
+        hipLaunchKernelGGL(kern, dim3(total), dim3(threads), 0, s, (const Args *)tab, S);                   \
+        MDF_LAUNCHED();                                                                                     \
+        return MDF_OK;                                                                                      \
+    }
+MDF_SEG_LAUNCH(launch_bin_count, k_bin_count_seg, 256, BinArgs)
+MDF_SEG_LAUNCH(launch_scan_tile_sums, k_scan_tile_sums_seg, 256, BinArgs)
+MDF_SEG_LAUNCH(launch_scan_tiles, k_scan_tiles_seg, 1024, BinArgs)
+MDF_SEG_LAUNCH(launch_scan_apply, k_scan_apply_seg, 256, BinArgs)
+MDF_SEG_LAUNCH(launch_bin_place, k_bin_place_seg, 256, BinArgs)
+#undef MDF_SEG_LAUNCH
+
+void seg_stages_bin(std::vector<SegStage> &out) {
+    out.push_back({"bin_count", sizeof(BinArgs), build_bin_particles, launch_bin_count});
+    out.push_back({"scan_tile_sums", sizeof(BinArgs), build_bin_tiles, launch_scan_tile_sums});
+    out.push_back({"scan_tiles", sizeof(BinArgs), build_bin_one, launch_scan_tiles});
+    out.push_back({"scan_apply", sizeof(BinArgs), build_bin_tiles, launch_scan_apply});
+    out.push_back({"bin_place", sizeof(BinArgs), build_bin_particles, launch_bin_place});
 }
 
 }  // namespace mdf

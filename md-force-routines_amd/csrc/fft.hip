@@ -94,14 +94,13 @@ __device__ C *stockham(C *a, C *b, int nl, int m, int lm, const C *__restrict__ 
 // Block: nl complex lines = 2*nl consecutive real rows.  The real input is
 // always the FP64 charge mesh; T is the arithmetic of the transform.
 template <class T>
-__global__ __launch_bounds__(256) void k_fft_x_r2c(const double *__restrict__ rho, typename Cx<T>::t *__restrict__ ck,
-                                                   int m, int lm, int Mxh, int nl,
-                                                   const typename Cx<T>::t *__restrict__ tw) {
+__device__ __forceinline__ void r2c_body(const double *__restrict__ rho, typename Cx<T>::t *__restrict__ ck, int m,
+                                         int lm, int Mxh, int nl, const typename Cx<T>::t *__restrict__ tw, int bx,
+                                         unsigned char *smraw) {
     using C = typename Cx<T>::t;
-    extern __shared__ __align__(16) unsigned char smraw[];
     C *sm = (C *)smraw;
     C *a = sm, *b = sm + nl * m;
-    const size_t row0 = (size_t)blockIdx.x * 2 * nl;
+    const size_t row0 = (size_t)bx * 2 * nl;
     for (int t = threadIdx.x; t < nl * m; t += blockDim.x) {
         const int l = t / m, x = t - l * m;
         const size_t ra = row0 + 2 * l;
@@ -120,18 +119,26 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const double *__restrict__ rh
     }
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_x_r2c(const double *__restrict__ rho, typename Cx<T>::t *__restrict__ ck,
+                                                   int m, int lm, int Mxh, int nl,
+                                                   const typename Cx<T>::t *__restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    r2c_body<T>(rho, ck, m, lm, Mxh, nl, tw, blockIdx.x, smraw);
+}
+
 // ---- passes 2 and 4: complex lines along a strided axis --------------------
 // Element e of line (o, kx) sits at o*os + e*es + kx; a block transforms kt
 // consecutive kx of one outer index o (blockIdx.y) in field blockIdx.z.
 template <int DIR, class C>
-__global__ __launch_bounds__(256) void k_fft_strided(C *__restrict__ data, size_t fstride, int m, int lm, int es, int os,
-                                                     int Mxh, int kt, const C *__restrict__ tw) {
-    extern __shared__ __align__(16) unsigned char smraw[];
+__device__ __forceinline__ void strided_body(C *__restrict__ data, size_t fstride, int m, int lm, int es, int os,
+                                             int Mxh, int kt, const C *__restrict__ tw, int bx, int by, int bz,
+                                             unsigned char *smraw) {
     C *sm = (C *)smraw;
-    const int kx0 = blockIdx.x * kt;
+    const int kx0 = bx * kt;
     const int nk = min(kt, Mxh - kx0);
     C *a = sm, *b = sm + kt * m;
-    C *base = data + blockIdx.z * fstride + (size_t)blockIdx.y * os + kx0;
+    C *base = data + bz * fstride + (size_t)by * os + kx0;
     for (int t = threadIdx.x; t < nk * m; t += blockDim.x) {
         const int e = t / nk, kk = t - e * nk;
         a[kk * m + e] = base[(size_t)e * es + kk];
@@ -142,6 +149,13 @@ __global__ __launch_bounds__(256) void k_fft_strided(C *__restrict__ data, size_
         const int e = t / nk, kk = t - e * nk;
         base[(size_t)e * es + kk] = r[kk * m + e];
     }
+}
+
+template <int DIR, class C>
+__global__ __launch_bounds__(256) void k_fft_strided(C *__restrict__ data, size_t fstride, int m, int lm, int es, int os,
+                                                     int Mxh, int kt, const C *__restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    strided_body<DIR, C>(data, fstride, m, lm, es, os, Mxh, kt, tw, blockIdx.x, blockIdx.y, blockIdx.z, smraw);
 }
 
 // ---- pass 3: z forward, Green, energy, ik, z inverse ----------------------
@@ -164,14 +178,13 @@ struct GreenZArgs {
 __device__ __forceinline__ double kval(int n, int M, double f) { return f * (double)(n < M / 2 ? n : n - M); }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fft_z_green(GreenZArgs<T> P) {
+__device__ __forceinline__ void zgreen_body(const GreenZArgs<T> &P, int bx, int by, int nbx, unsigned char *smraw,
+                                            double *red) {
     using C = typename Cx<T>::t;
-    extern __shared__ __align__(16) unsigned char smraw[];
     C *sm = (C *)smraw;
-    __shared__ double red[4];
     const int m = P.M[2], My = P.M[1], Mxh = P.Mxh, kt = P.kt;
-    const int y = blockIdx.y;
-    const int kx0 = blockIdx.x * kt;
+    const int y = by;
+    const int kx0 = bx * kt;
     const int nk = min(kt, Mxh - kx0);
     const int es = My * Mxh;
     const size_t off = (size_t)y * Mxh + kx0;
@@ -196,7 +209,7 @@ __global__ __launch_bounds__(256) void k_fft_z_green(GreenZArgs<T> P) {
     esum = wave_sum(esum);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = esum;
     __syncthreads();
-    if (threadIdx.x == 0) P.part[blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) P.part[by * nbx + bx] = red[0] + red[1] + red[2] + red[3];
     for (int ax = 0; ax < 3; ++ax) {
         for (int t = threadIdx.x; t < nk * m; t += blockDim.x) {
             const int kk = t / m, kz = t - kk * m;
@@ -215,6 +228,13 @@ __global__ __launch_bounds__(256) void k_fft_z_green(GreenZArgs<T> P) {
         }
         __syncthreads();
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_z_green(GreenZArgs<T> P) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    __shared__ double red[4];
+    zgreen_body<T>(P, blockIdx.x, blockIdx.y, gridDim.x, smraw, red);
 }
 
 // ---- pass 5: x, complex to real ------------------------------------------
@@ -236,15 +256,14 @@ __device__ __forceinline__ C herm_get(const C *__restrict__ row, int k, int m) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fft_x_c2r(const typename Cx<T>::t *__restrict__ ek, T *__restrict__ emesh,
-                                                   int m, int lm, int Mxh, int np, size_t nspec,
-                                                   const typename Cx<T>::t *__restrict__ tw) {
+__device__ __forceinline__ void c2r_body(const typename Cx<T>::t *__restrict__ ek, T *__restrict__ emesh, int m, int lm,
+                                         int Mxh, int np, size_t nspec, const typename Cx<T>::t *__restrict__ tw,
+                                         int bx, unsigned char *smraw) {
     using C = typename Cx<T>::t;
-    extern __shared__ __align__(16) unsigned char smraw[];
     C *sm = (C *)smraw;
     const int nl = 3 * np;
     C *a = sm, *b = sm + nl * m;
-    const size_t R0 = (size_t)blockIdx.x * 2 * np;
+    const size_t R0 = (size_t)bx * 2 * np;
     for (int t = threadIdx.x; t < np * m; t += blockDim.x) {
         const int p = t / m, k = t - p * m;
         const size_t R = R0 + 2 * p;
@@ -279,10 +298,98 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const typename Cx<T>::t *__re
     }
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_x_c2r(const typename Cx<T>::t *__restrict__ ek, T *__restrict__ emesh,
+                                                   int m, int lm, int Mxh, int np, size_t nspec,
+                                                   const typename Cx<T>::t *__restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    c2r_body<T>(ek, emesh, m, lm, Mxh, np, nspec, tw, blockIdx.x, smraw);
+}
+
 static int ilog2(int m) {
     int l = 0;
     while ((1 << l) < m) ++l;
     return l;
+}
+
+// Parameters of the five passes of one solve.
+template <class T>
+struct FftPlan {
+    using C = typename Cx<T>::t;
+    const double *rho;
+    C *ck, *ek;
+    T *emesh;
+    const C *tw0, *tw1, *tw2;
+    int Mx, My, Mz, Mxh, lx, ly, lz;
+    size_t nspec;
+    int nl, g1;          // pass 1: lines per block, blocks
+    size_t lds1;
+    int kty, g2x;        // passes 2 and 4
+    size_t lds2;
+    GreenZArgs<T> gz;    // pass 3
+    int g3x;
+    size_t lds3;
+    int np, g5;          // pass 5
+    size_t lds5;
+};
+
+template <class T>
+static FftPlan<T> fft_plan(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const typename Cx<T>::t *tw1,
+                           const typename Cx<T>::t *tw2, typename Cx<T>::t *ck, typename Cx<T>::t *ek, const T *green,
+                           T *emesh) {
+    using C = typename Cx<T>::t;
+    FftPlan<T> F;
+    F.rho = c->rho;
+    F.ck = ck;
+    F.ek = ek;
+    F.emesh = emesh;
+    F.tw0 = tw0;
+    F.tw1 = tw1;
+    F.tw2 = tw2;
+    const int Mx = c->g.M[0], My = c->g.M[1], Mz = c->g.M[2], Mxh = c->g.Mxh;
+    F.Mx = Mx;
+    F.My = My;
+    F.Mz = Mz;
+    F.Mxh = Mxh;
+    F.lx = ilog2(Mx);
+    F.ly = ilog2(My);
+    F.lz = ilog2(Mz);
+    F.nspec = c->nspec;
+    const size_t cs = sizeof(C);
+    int nl = 512 / Mx;
+    if (nl < 1) nl = 1;
+    while (nl > 1 && (My * Mz) % (2 * nl)) nl >>= 1;
+    F.nl = nl;
+    F.g1 = (My * Mz) / (2 * nl);
+    F.lds1 = 2 * cs * nl * Mx;
+    F.kty = My >= 512 ? 4 : (My >= 256 ? 8 : 16);
+    F.g2x = (Mxh + F.kty - 1) / F.kty;
+    F.lds2 = 2 * cs * F.kty * My;
+    GreenZArgs<T> &P = F.gz;
+    P.ck = ck;
+    P.ek = ek;
+    P.green = green;
+    P.tw = tw2;
+    P.part = c->part_pm;
+    P.nspec = c->nspec;
+    P.M[0] = Mx;
+    P.M[1] = My;
+    P.M[2] = Mz;
+    P.Mxh = Mxh;
+    P.lm = F.lz;
+    P.kt = Mz >= 512 ? 2 : (Mz >= 256 ? 4 : 8);
+    if (sizeof(T) == 4) P.kt *= 2;
+    for (int d = 0; d < 3; ++d) P.kfac[d] = 2.0 * M_PI / c->g.L[d];
+    P.inv_V = 1.0 / (c->g.L[0] * c->g.L[1] * c->g.L[2]);
+    F.g3x = (Mxh + P.kt - 1) / P.kt;
+    F.lds3 = 3 * cs * P.kt * Mz;
+    int np = 256 / Mx;
+    if (np < 1) np = 1;
+    while (np > 1 && (My * Mz) % (2 * np)) np >>= 1;
+    F.np = np;
+    F.g5 = (My * Mz) / (2 * np);
+    F.lds5 = 2 * cs * 3 * np * Mx;
+    return F;
 }
 
 template <class T>
@@ -290,71 +397,115 @@ static int pm_solve_t(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const typena
                       const typename Cx<T>::t *tw2, typename Cx<T>::t *ck, typename Cx<T>::t *ek, const T *green,
                       T *emesh, hipStream_t s) {
     using C = typename Cx<T>::t;
-    const int Mx = c->g.M[0], My = c->g.M[1], Mz = c->g.M[2], Mxh = c->g.Mxh;
-    const int lx = ilog2(Mx), ly = ilog2(My), lz = ilog2(Mz);
-    const size_t cs = sizeof(C);
-    // pass 1
-    {
-        int nl = 512 / Mx;
-        if (nl < 1) nl = 1;
-        while (nl > 1 && (My * Mz) % (2 * nl)) nl >>= 1;
-        const size_t lds = 2 * cs * nl * Mx;
-        hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3((My * Mz) / (2 * nl)), dim3(256), lds, s, c->rho, ck, Mx, lx, Mxh, nl,
-                           tw0);
-        MDF_LAUNCHED();
+    const FftPlan<T> F = fft_plan<T>(c, tw0, tw1, tw2, ck, ek, green, emesh);
+    hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(F.g1), dim3(256), F.lds1, s, c->rho, ck, F.Mx, F.lx, F.Mxh, F.nl, tw0);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL((k_fft_strided<-1, C>), dim3(F.g2x, F.Mz, 1), dim3(256), F.lds2, s, ck, (size_t)0, F.My,
+                       F.ly, F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
+    MDF_LAUNCHED();
+    c->n_part_pm = F.g3x * F.My;
+    hipLaunchKernelGGL(k_fft_z_green<T>, dim3(F.g3x, F.My), dim3(256), F.lds3, s, F.gz);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL((k_fft_strided<1, C>), dim3(F.g2x, F.Mz, 3), dim3(256), F.lds2, s, ek, c->nspec, F.My, F.ly,
+                       F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(F.g5), dim3(256), F.lds5, s, ek, emesh, F.Mx, F.lx, F.Mxh, F.np,
+                       c->nspec, tw0);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+// ---- segmented stages (replica batches) ----
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_r2c_seg(const FftPlan<T> *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    Blk b;
+    const FftPlan<T> &A = tab[seg_locate(S, b)];
+    r2c_body<T>(A.rho, A.ck, A.Mx, A.lx, A.Mxh, A.nl, A.tw0, b.x, smraw);
+}
+template <int DIR, class T>
+__global__ __launch_bounds__(256) void k_fft_y_seg(const FftPlan<T> *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    Blk b;
+    const FftPlan<T> &A = tab[seg_locate(S, b)];
+    strided_body<DIR, typename Cx<T>::t>(DIR < 0 ? A.ck : A.ek, DIR < 0 ? (size_t)0 : A.nspec, A.My, A.ly, A.Mxh,
+                                         A.My * A.Mxh, A.Mxh, A.kty, A.tw1, b.x, b.y, b.z, smraw);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_zgreen_seg(const FftPlan<T> *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    __shared__ double red[4];
+    Blk b;
+    const FftPlan<T> &A = tab[seg_locate(S, b)];
+    zgreen_body<T>(A.gz, b.x, b.y, b.nx, smraw, red);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_c2r_seg(const FftPlan<T> *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    Blk b;
+    const FftPlan<T> &A = tab[seg_locate(S, b)];
+    c2r_body<T>(A.ek, A.emesh, A.Mx, A.lx, A.Mxh, A.np, A.nspec, A.tw0, b.x, smraw);
+}
+
+template <class T>
+static FftPlan<T> ctx_plan(mdf_p3m_ctx *c) {
+    if (sizeof(T) == 4)
+        return fft_plan<T>(c, (const typename Cx<T>::t *)c->twf[0], (const typename Cx<T>::t *)c->twf[1],
+                           (const typename Cx<T>::t *)c->twf[2], (typename Cx<T>::t *)c->ckf,
+                           (typename Cx<T>::t *)c->ekf, (const T *)c->greenf, (T *)c->emeshf);
+    return fft_plan<T>(c, (const typename Cx<T>::t *)c->tw[0], (const typename Cx<T>::t *)c->tw[1],
+                       (const typename Cx<T>::t *)c->tw[2], (typename Cx<T>::t *)c->ck, (typename Cx<T>::t *)c->ek,
+                       (const T *)c->green, (T *)c->emesh);
+}
+
+template <class T, int PASS>
+static int build_fft(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size_t &lds) {
+    if ((sizeof(T) == 4) != (c->prm.precision == MDF_PREC_MIXED)) {
+        set_error("replica batch: mixed precisions");
+        return MDF_EINVAL;
     }
-    // pass 2: y forward
-    const int kty = My >= 512 ? 4 : (My >= 256 ? 8 : 16);
-    {
-        const size_t lds = 2 * cs * kty * My;
-        dim3 grid((Mxh + kty - 1) / kty, Mz, 1);
-        hipLaunchKernelGGL((k_fft_strided<-1, C>), grid, dim3(256), lds, s, ck, (size_t)0, My, ly, Mxh, My * Mxh,
-                           Mxh, kty, tw1);
-        MDF_LAUNCHED();
-    }
-    // pass 3: z + Green + ik
-    {
-        GreenZArgs<T> P;
-        P.ck = ck;
-        P.ek = ek;
-        P.green = green;
-        P.tw = tw2;
-        P.part = c->part_pm;
-        P.nspec = c->nspec;
-        P.M[0] = Mx;
-        P.M[1] = My;
-        P.M[2] = Mz;
-        P.Mxh = Mxh;
-        P.lm = lz;
-        P.kt = Mz >= 512 ? 2 : (Mz >= 256 ? 4 : 8);
-        if (sizeof(T) == 4) P.kt *= 2;
-        for (int d = 0; d < 3; ++d) P.kfac[d] = 2.0 * M_PI / c->g.L[d];
-        P.inv_V = 1.0 / (c->g.L[0] * c->g.L[1] * c->g.L[2]);
-        const size_t lds = 3 * cs * P.kt * Mz;
-        dim3 grid((Mxh + P.kt - 1) / P.kt, My, 1);
-        c->n_part_pm = grid.x * grid.y;
-        hipLaunchKernelGGL(k_fft_z_green<T>, grid, dim3(256), lds, s, P);
-        MDF_LAUNCHED();
-    }
-    // pass 4: y inverse, 3 fields
-    {
-        const size_t lds = 2 * cs * kty * My;
-        dim3 grid((Mxh + kty - 1) / kty, Mz, 3);
-        hipLaunchKernelGGL((k_fft_strided<1, C>), grid, dim3(256), lds, s, ek, c->nspec, My, ly, Mxh, My * Mxh, Mxh,
-                           kty, tw1);
-        MDF_LAUNCHED();
-    }
-    // pass 5: x complex to real
-    {
-        int np = 256 / Mx;
-        if (np < 1) np = 1;
-        while (np > 1 && (My * Mz) % (2 * np)) np >>= 1;
-        const size_t lds = 2 * cs * 3 * np * Mx;
-        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3((My * Mz) / (2 * np)), dim3(256), lds, s, ek, emesh, Mx, lx, Mxh, np,
-                           c->nspec, tw0);
-        MDF_LAUNCHED();
+    const FftPlan<T> F = ctx_plan<T>(c);
+    *(FftPlan<T> *)args = F;
+    switch (PASS) {
+        case 1: dims = make_int4(F.g1, 1, 1, 0); lds = F.lds1; break;
+        case 2: dims = make_int4(F.g2x, F.Mz, 1, 0); lds = F.lds2; break;
+        case 3:
+            dims = make_int4(F.g3x, F.My, 1, 0);
+            lds = F.lds3;
+            c->n_part_pm = F.g3x * F.My;
+            break;
+        case 4: dims = make_int4(F.g2x, F.Mz, 3, 0); lds = F.lds2; break;
+        default: dims = make_int4(F.g5, 1, 1, 0); lds = F.lds5; break;
     }
     return MDF_OK;
+}
+
+template <class T, int PASS>
+static int launch_fft(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
+    const FftPlan<T> *t = (const FftPlan<T> *)tab;
+    switch (PASS) {
+        case 1: hipLaunchKernelGGL(k_fft_r2c_seg<T>, dim3(total), dim3(256), lds, s, t, S); break;
+        case 2: hipLaunchKernelGGL((k_fft_y_seg<-1, T>), dim3(total), dim3(256), lds, s, t, S); break;
+        case 3: hipLaunchKernelGGL(k_fft_zgreen_seg<T>, dim3(total), dim3(256), lds, s, t, S); break;
+        case 4: hipLaunchKernelGGL((k_fft_y_seg<1, T>), dim3(total), dim3(256), lds, s, t, S); break;
+        default: hipLaunchKernelGGL(k_fft_c2r_seg<T>, dim3(total), dim3(256), lds, s, t, S); break;
+    }
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+template <class T>
+static void push_fft(std::vector<SegStage> &out) {
+    out.push_back({"fft_r2c", sizeof(FftPlan<T>), build_fft<T, 1>, launch_fft<T, 1>});
+    out.push_back({"fft_y_fwd", sizeof(FftPlan<T>), build_fft<T, 2>, launch_fft<T, 2>});
+    out.push_back({"fft_z_green", sizeof(FftPlan<T>), build_fft<T, 3>, launch_fft<T, 3>});
+    out.push_back({"fft_y_inv", sizeof(FftPlan<T>), build_fft<T, 4>, launch_fft<T, 4>});
+    out.push_back({"fft_c2r", sizeof(FftPlan<T>), build_fft<T, 5>, launch_fft<T, 5>});
+}
+
+void seg_stages_fft(std::vector<SegStage> &out, int precision) {
+    if (precision == MDF_PREC_MIXED) push_fft<float>(out);
+    else push_fft<double>(out);
 }
 
 int pm_solve(mdf_p3m_ctx *c, hipStream_t s) {

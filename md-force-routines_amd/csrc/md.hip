@@ -38,11 +38,26 @@ struct WrapBox {
     double L[3], lo[3], hi[3];
 };
 
-__global__ __launch_bounds__(256) void k_kick_drift(double *__restrict__ pos, double *__restrict__ vel,
-                                                    const double *__restrict__ acc, int n3, double dt, int pbc,
-                                                    WrapBox wb, const double *__restrict__ scale) {
+struct KickDriftArgs {
+    double *pos, *vel;
+    const double *acc;
+    int n3;
+    double dt;
+    int pbc;
+    WrapBox wb;
+    const double *scale;
+};
+
+__device__ __forceinline__ void kick_drift_body(const KickDriftArgs &A, const Blk &b) {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double *__restrict__ pos = A.pos;
+    double *__restrict__ vel = A.vel;
+    const double *__restrict__ acc = A.acc;
+    const double *__restrict__ scale = A.scale;
+    const int n3 = A.n3, pbc = A.pbc;
+    const double dt = A.dt;
+    const WrapBox &wb = A.wb;
+    const int i = b.x * blockDim.x + threadIdx.x;
     if (i >= n3) return;
     const int d = i % 3;
     // separate roundings (no FMA contraction), as the reference's NumPy expressions
@@ -56,6 +71,13 @@ __global__ __launch_bounds__(256) void k_kick_drift(double *__restrict__ pos, do
     }
     vel[i] = v;
     pos[i] = x;
+}
+
+__global__ __launch_bounds__(256) void k_kick_drift(KickDriftArgs A) { kick_drift_body(A, blk_plain()); }
+__global__ __launch_bounds__(256) void k_kick_drift_seg(const KickDriftArgs *__restrict__ tab, SegGrid S) {
+    Blk b;
+    const int r = seg_locate(S, b);
+    kick_drift_body(tab[r], b);
 }
 
 __global__ __launch_bounds__(256) void k_kick(double *__restrict__ vel, const double *__restrict__ acc, int n3,
@@ -135,19 +157,30 @@ __global__ __launch_bounds__(256) void k_modes(const double *__restrict__ pos, i
 // the layout k_modes_final sums.
 constexpr int kModeChunk = 16;
 
-__global__ __launch_bounds__(256) void k_modes_grid(const double *__restrict__ pos, int n, double dq, int nq,
-                                                    double *__restrict__ part) {
-    // blockIdx.y: axis, blockIdx.z: chunk of kModeChunk modes starting at m0
+struct ModesArgs {
+    const double *pos;
+    int n;
+    double dq;
+    int nq;
+    double *part;
+};
+
+__device__ __forceinline__ void modes_grid_body(const ModesArgs &A, const Blk &b) {
+    // b.y: axis, b.z: chunk of kModeChunk modes starting at m0
+    const double *__restrict__ pos = A.pos;
+    double *__restrict__ part = A.part;
+    const int n = A.n, nq = A.nq;
+    const double dq = A.dq;
     const int lane = threadIdx.x & 63;
-    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    const int stride = gridDim.x * blockDim.x;
-    const int a = blockIdx.y;
-    const int m0 = blockIdx.z * kModeChunk;
+    const int gw = (b.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (b.nx * blockDim.x) >> 6;
+    const int stride = b.nx * blockDim.x;
+    const int a = b.y;
+    const int m0 = b.z * kModeChunk;
     double ar[kModeChunk], ai[kModeChunk];
 #pragma unroll
     for (int k = 0; k < kModeChunk; ++k) ar[k] = ai[k] = 0.0;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    for (int j = b.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         const double th = dq * pos[3 * j + a];
         double s1, c1, zs, zc;
         sincos(-th, &s1, &c1);
@@ -181,17 +214,42 @@ __global__ __launch_bounds__(256) void k_modes_grid(const double *__restrict__ p
     }
 }
 
-// Sum the per-wave partials of k_modes_grid: block m, out[m * 6 + c].
-__global__ __launch_bounds__(256) void k_modes_grid_final(const double *__restrict__ part, int nw,
-                                                          double *__restrict__ out) {
-    __shared__ double red[4];
-    const int m = blockIdx.x;
+__global__ __launch_bounds__(256) void k_modes_grid(ModesArgs A) { modes_grid_body(A, blk_plain()); }
+__global__ __launch_bounds__(256) void k_modes_grid_seg(const ModesArgs *__restrict__ tab, SegGrid S) {
+    Blk b;
+    const int r = seg_locate(S, b);
+    modes_grid_body(tab[r], b);
+}
+
+// Sum the per-wave partials of k_modes_grid: block m, out[k * out_stride + m * 6 + c].
+struct ModesFinalArgs {
+    const double *part;
+    int nw;
+    double *out;
+    int out_stride;
+};
+
+__device__ __forceinline__ void modes_final_body(const ModesFinalArgs &A, const Blk &b, int k, double *red) {
+    const int m = b.x;
+    double *out = A.out + (size_t)k * A.out_stride;
     for (int c = 0; c < 6; ++c) {
         double v = 0.0;
-        for (int w = threadIdx.x; w < nw; w += blockDim.x) v += part[((size_t)m * nw + w) * 6 + c];
+        for (int w = threadIdx.x; w < A.nw; w += blockDim.x) v += A.part[((size_t)m * A.nw + w) * 6 + c];
         v = block_sum_256(v, red);
         if (threadIdx.x == 0) out[(size_t)m * 6 + c] = v;
     }
+}
+
+__global__ __launch_bounds__(256) void k_modes_grid_final(ModesFinalArgs A) {
+    __shared__ double red[4];
+    modes_final_body(A, blk_plain(), 0, red);
+}
+__global__ __launch_bounds__(256) void k_modes_grid_final_seg(const ModesFinalArgs *__restrict__ tab, SegGrid S,
+                                                              int k) {
+    __shared__ double red[4];
+    Blk b;
+    const int r = seg_locate(S, b);
+    modes_final_body(tab[r], b, k, red);
 }
 
 __global__ __launch_bounds__(64) void k_modes_final(const double *__restrict__ part, int nblk, int nq,
@@ -217,11 +275,11 @@ struct FinalArgs {
     int n;
     double *scal;
     double *energies;  // 4 or null
-    double *rec;       // 4 or null
+    double *rec;       // 4 or null (segmented launches: step k writes rec + k * rec_stride)
+    int rec_stride;
 };
 
-__global__ __launch_bounds__(1024) void k_finalize(FinalArgs P) {
-    __shared__ double red[16];
+__device__ __forceinline__ void finalize_body(const FinalArgs &P, int step, double *red) {
     double a = 0.0, b = 0.0, k = 0.0;
     for (int i = threadIdx.x; i < P.npp; i += blockDim.x) a += P.pp[i];
     for (int i = threadIdx.x; i < P.npm; i += blockDim.x) b += P.pm[i];
@@ -248,13 +306,25 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalArgs P) {
             P.scal[kScalK] = K;
             P.scal[kScalT] = K / P.kf / (double)P.n;
             if (P.rec) {
-                P.rec[0] = K + U;
-                P.rec[1] = K;
-                P.rec[2] = U;
-                P.rec[3] = K / P.kf / (double)P.n;
+                double *rec = P.rec + (size_t)step * P.rec_stride;
+                rec[0] = K + U;
+                rec[1] = K;
+                rec[2] = U;
+                rec[3] = K / P.kf / (double)P.n;
             }
         }
     }
+}
+
+__global__ __launch_bounds__(1024) void k_finalize(FinalArgs P) {
+    __shared__ double red[16];
+    finalize_body(P, 0, red);
+}
+__global__ __launch_bounds__(1024) void k_finalize_seg(const FinalArgs *__restrict__ tab, SegGrid S, int k) {
+    __shared__ double red[16];
+    Blk b;
+    const int r = seg_locate(S, b);
+    finalize_body(tab[r], k, red);
 }
 
 int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, double *energies, hipStream_t s) {
@@ -278,7 +348,7 @@ int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, 
     return MDF_OK;
 }
 
-static int finalize_step(mdf_p3m_ctx *c, int n, double uf, double kf, double *rec, hipStream_t s) {
+static FinalArgs finalize_step_args(mdf_p3m_ctx *c, int n, double uf, double kf, double *rec) {
     FinalArgs P{};
     P.pp = c->part_pp;
     P.npp = c->nblock;
@@ -294,6 +364,12 @@ static int finalize_step(mdf_p3m_ctx *c, int n, double uf, double kf, double *re
     P.scal = c->scal;
     P.energies = nullptr;
     P.rec = rec;
+    P.rec_stride = 0;
+    return P;
+}
+
+static int finalize_step(mdf_p3m_ctx *c, int n, double uf, double kf, double *rec, hipStream_t s) {
+    const FinalArgs P = finalize_step_args(c, n, uf, kf, rec);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, P);
     MDF_LAUNCHED();
     return MDF_OK;
@@ -309,8 +385,8 @@ int kick_drift(mdf_p3m_ctx *c, double *pos, double *vel, const double *acc, int 
         wb.lo[d] = lmin ? lmin[d] : 0.0;
         wb.hi[d] = lmax ? lmax[d] : c->g.L[d];
     }
-    const int n3 = 3 * n;
-    hipLaunchKernelGGL(k_kick_drift, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, vel, acc, n3, dt, pbc, wb, scale);
+    KickDriftArgs A{pos, vel, acc, 3 * n, dt, pbc, wb, scale};
+    hipLaunchKernelGGL(k_kick_drift, dim3((A.n3 + 255) / 256), dim3(256), 0, s, A);
     MDF_LAUNCHED();
     return MDF_OK;
 }
@@ -356,9 +432,11 @@ int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int 
     if (nb < 1) nb = 1;
     const int nw = nb * 4;
     const int nchunk = (nq + kModeChunk - 1) / kModeChunk;
-    hipLaunchKernelGGL(k_modes_grid, dim3(nb, 3, nchunk), dim3(256), 0, s, pos, n, dq, nq, c->part_red);
+    const ModesArgs A{pos, n, dq, nq, c->part_red};
+    hipLaunchKernelGGL(k_modes_grid, dim3(nb, 3, nchunk), dim3(256), 0, s, A);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_modes_grid_final, dim3(nq), dim3(256), 0, s, c->part_red, nw, out);
+    const ModesFinalArgs F{c->part_red, nw, out, 0};
+    hipLaunchKernelGGL(k_modes_grid_final, dim3(nq), dim3(256), 0, s, F);
     MDF_LAUNCHED();
     return MDF_OK;
 }
@@ -398,6 +476,92 @@ int md_step(mdf_p3m_ctx *c, double *pos, double *vel, double *acc, const double 
     prof_mark(c, kStageFinal, s);
     MDF_TRY(finalize_step(c, n, uf, kf, rec, s));
     return MDF_OK;
+}
+
+// ---- segmented stages (replica batches) ----
+static int build_drift(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    WrapBox wb;
+    for (int d = 0; d < 3; ++d) {
+        wb.L[d] = c->g.L[d];
+        wb.lo[d] = 0.0;
+        wb.hi[d] = c->g.L[d];
+    }
+    *(KickDriftArgs *)args = KickDriftArgs{io.pos, io.vel, io.acc, 3 * io.n, io.dt, 1, wb, nullptr};
+    const int nb = (3 * io.n + 255) / 256;
+    dims = make_int4(nb > 0 ? nb : 1, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int launch_drift(const void *tab, const SegGrid &S, int total, size_t, int, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick_drift_seg, dim3(total), dim3(256), 0, s, (const KickDriftArgs *)tab, S);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+static int build_final(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    FinalArgs P = finalize_step_args(c, io.n, io.uf, io.kf, io.rec0);
+    P.rec_stride = io.rec_stride;
+    *(FinalArgs *)args = P;
+    dims = make_int4(1, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int launch_final(const void *tab, const SegGrid &S, int total, size_t, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize_seg, dim3(total), dim3(1024), 0, s, (const FinalArgs *)tab, S, k);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+static int modes_blocks(int n) {
+    int nb = (n + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    return nb < 1 ? 1 : nb;
+}
+static int build_modes(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    if (io.nq > 64) {
+        set_error("density modes on a q grid: nq = %d > 64", io.nq);
+        return MDF_EINVAL;
+    }
+    *(ModesArgs *)args = ModesArgs{io.pos, io.n, 2.0 * M_PI / c->g.L[0], io.nq, c->part_red};
+    dims = make_int4(modes_blocks(io.n), 3, (io.nq + kModeChunk - 1) / kModeChunk, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int launch_modes(const void *tab, const SegGrid &S, int total, size_t, int, hipStream_t s) {
+    hipLaunchKernelGGL(k_modes_grid_seg, dim3(total), dim3(256), 0, s, (const ModesArgs *)tab, S);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+static int build_modes_final(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    *(ModesFinalArgs *)args = ModesFinalArgs{c->part_red, modes_blocks(io.n) * 4, io.modes0, io.modes_stride};
+    dims = make_int4(io.nq, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int launch_modes_final(const void *tab, const SegGrid &S, int total, size_t, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_modes_grid_final_seg, dim3(total), dim3(256), 0, s, (const ModesFinalArgs *)tab, S, k);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+void seg_stages_md(std::vector<SegStage> &out, int part) {
+    if (part == 0) out.push_back({"drift", sizeof(KickDriftArgs), build_drift, launch_drift});
+    if (part == 1) out.push_back({"finalize", sizeof(FinalArgs), build_final, launch_final});
+    if (part == 2) {
+        out.push_back({"modes", sizeof(ModesArgs), build_modes, launch_modes});
+        out.push_back({"modes_final", sizeof(ModesFinalArgs), build_modes_final, launch_modes_final});
+    }
+}
+
+int seg_step_stages(int precision, std::vector<SegStage> &out) {
+    out.clear();
+    seg_stages_md(out, 0);
+    seg_stages_bin(out);
+    seg_stages_pp(out, precision);
+    seg_stages_pm(out, precision, 0);
+    seg_stages_fft(out, precision);
+    seg_stages_pm(out, precision, 1);
+    seg_stages_md(out, 1);
+    seg_stages_md(out, 2);
+    return (int)out.size();
 }
 
 }  // namespace mdf

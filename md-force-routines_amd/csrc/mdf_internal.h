@@ -210,6 +210,65 @@ inline int allow_lds(F *kernel, size_t bytes) {
     return MDF_OK;
 }
 
+// Block coordinates seen by a kernel body.  A plain launch passes its own
+// blockIdx/gridDim; a segmented launch (one launch for every replica of a
+// batch) passes the block's place inside its replica's grid.
+struct Blk {
+    int x, y, z, nx, ny, nz;
+};
+__device__ __forceinline__ Blk blk_plain() {
+    return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+// Segmented grids: replica r owns the 1-D blocks [first[r], first[r+1]) and
+// its own grid dims[r] (flattened x fastest).
+struct SegGrid {
+    const int *first;
+    const int4 *dims;  // (nx, ny, nz, -)
+    int nrep;
+};
+__device__ __forceinline__ int seg_locate(const SegGrid &S, Blk &b) {
+    const int g = (int)blockIdx.x;
+    int lo = 0, hi = S.nrep - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.first[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const int l = g - S.first[lo];
+    const int4 d = S.dims[lo];
+    b = Blk{l % d.x, (l / d.x) % d.y, l / (d.x * d.y), d.x, d.y, d.z};
+    return lo;
+}
+
+// Segmented form of the fused NVE step (replica batches).
+// Per-replica inputs of one step; per-step records of step k go to
+// rec0 + k * rec_stride and modes0 + k * modes_stride.
+struct StepIO {
+    double *pos, *vel, *acc;
+    int n;
+    double dt, af, uf, kf;
+    double *rec0;
+    int rec_stride;
+    double *modes0;
+    int modes_stride;
+    int nq;
+};
+// One stage: build() fills the replica's argument block, its grid and its
+// dynamic LDS bytes; launch() runs the stage for all replicas at once.
+struct SegStage {
+    const char *name;
+    size_t arg_size;
+    int (*build)(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds);
+    int (*launch)(const void *tab, const SegGrid &S, int total, size_t lds, int k, hipStream_t s);
+};
+// Stages of md_step followed by the n(q) grid modes, in order, for one precision.
+int seg_step_stages(int precision, std::vector<SegStage> &out);
+void seg_stages_md(std::vector<SegStage> &out, int part);    // part 0: drift; 1: finalize; 2: modes
+void seg_stages_bin(std::vector<SegStage> &out);
+void seg_stages_pp(std::vector<SegStage> &out, int precision);
+void seg_stages_pm(std::vector<SegStage> &out, int precision, int part);  // 0: fill + assign, 1: gather
+void seg_stages_fft(std::vector<SegStage> &out, int precision);
+
 // Wave (64-lane) helpers.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

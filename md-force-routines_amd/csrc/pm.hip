@@ -129,14 +129,13 @@ struct AssignArgs {
 };
 
 template <bool LDS, class T>
-__global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
-    extern __shared__ __align__(16) unsigned char braw[];
+__device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, unsigned char *braw, int *run_start,
+                                            int *run_off) {
     double *brick = (double *)braw;  // FP64 in both modes: FP32 LDS atomics are ~6x slower here
-    __shared__ int run_start[64], run_off[65];
     const Geom &g = P.g;
     const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
     const int nb = B0 * B1 * B2;
-    const PMBlock blk = pm_block(g, P.br.cb_assign, blockIdx.x);
+    const PMBlock blk = pm_block(g, P.br.cb_assign, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     for (int t = threadIdx.x; t < nb; t += blockDim.x) brick[t] = 0.0;
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
@@ -190,9 +189,23 @@ __global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
     }
 }
 
-int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
-    (void)n;
-    MDF_TRY(fill_zero(c->rho, c->nmesh, s));
+template <bool LDS, class T>
+__global__ __launch_bounds__(kPMThreads) void k_assign(AssignArgs P) {
+    extern __shared__ __align__(16) unsigned char braw[];
+    __shared__ int run_start[64], run_off[65];
+    assign_body<LDS, T>(P, blk_plain(), braw, run_start, run_off);
+}
+template <class T>
+__global__ __launch_bounds__(kPMThreads) void k_assign_seg(const AssignArgs *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char braw[];
+    __shared__ int run_start[64], run_off[65];
+    Blk b;
+    const int r = seg_locate(S, b);
+    if (tab[r].br.lds_assign) assign_body<true, T>(tab[r], b, braw, run_start, run_off);
+    else assign_body<false, T>(tab[r], b, braw, run_start, run_off);
+}
+
+static AssignArgs assign_args(mdf_p3m_ctx *c) {
     AssignArgs P;
     P.g = c->g;
     P.br = c->brick;
@@ -200,6 +213,13 @@ int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
     P.cell_start = c->cell_start;
     P.rho = c->rho;
     P.stats = c->stats;
+    return P;
+}
+
+int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
+    (void)n;
+    MDF_TRY(fill_zero(c->rho, c->nmesh, s));
+    const AssignArgs P = assign_args(c);
     const bool mixed = c->prm.precision == MDF_PREC_MIXED;
     const dim3 grid(c->brick.nblock_assign);
     if (c->brick.lds_assign) {
@@ -236,15 +256,13 @@ struct GatherArgs {
 };
 
 template <bool LDS, class T>
-__global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
-    extern __shared__ __align__(16) unsigned char eraw[];
+__device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, unsigned char *eraw, double *red,
+                                            int *run_start, int *run_off) {
     T *ebrick = (T *)eraw;
     const T *emesh = (const T *)P.emesh;
-    __shared__ double red[kGatherThreads / 64];
-    __shared__ int run_start[64], run_off[65];
     const Geom &g = P.g;
     const int B0 = P.br.Bg[0], B1 = P.br.Bg[1], B2 = P.br.Bg[2];
-    const PMBlock blk = pm_block(g, P.br.cb_gather, blockIdx.x);
+    const PMBlock blk = pm_block(g, P.br.cb_gather, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     if (LDS) {
         const int nb = B0 * B1 * B2;
@@ -377,13 +395,30 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
         if (threadIdx.x == 0) {
             double t = 0.0;
             for (int w = 0; w < kGatherThreads / 64; ++w) t += red[w];
-            P.part[blockIdx.x] = t;
+            P.part[bk.x] = t;
         }
     }
 }
 
-int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
-    (void)n;
+template <bool LDS, class T>
+__global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
+    extern __shared__ __align__(16) unsigned char eraw[];
+    __shared__ double red[kGatherThreads / 64];
+    __shared__ int run_start[64], run_off[65];
+    gather_body<LDS, T>(P, blk_plain(), eraw, red, run_start, run_off);
+}
+template <class T>
+__global__ __launch_bounds__(kGatherThreads) void k_gather_seg(const GatherArgs *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char eraw[];
+    __shared__ double red[kGatherThreads / 64];
+    __shared__ int run_start[64], run_off[65];
+    Blk b;
+    const int r = seg_locate(S, b);
+    if (tab[r].br.lds_gather) gather_body<true, T>(tab[r], b, eraw, red, run_start, run_off);
+    else gather_body<false, T>(tab[r], b, eraw, red, run_start, run_off);
+}
+
+static GatherArgs gather_args(mdf_p3m_ctx *c, const GatherOut &o) {
     GatherArgs P;
     P.g = c->g;
     P.br = c->brick;
@@ -395,6 +430,13 @@ int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
     P.o = o;
     const bool mixed = c->prm.precision == MDF_PREC_MIXED;
     P.emesh = mixed ? (const void *)c->emeshf : (const void *)c->emesh;
+    return P;
+}
+
+int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
+    (void)n;
+    const GatherArgs P = gather_args(c, o);
+    const bool mixed = c->prm.precision == MDF_PREC_MIXED;
     const dim3 grid(c->brick.nblock_gather);
     if (c->brick.lds_gather) {
         const size_t lds = 3 * (mixed ? 4 : 8) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2];
@@ -412,6 +454,77 @@ int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
     }
     MDF_LAUNCHED();
     return MDF_OK;
+}
+
+// ---- segmented stages (replica batches) ----
+struct FillArgs {
+    double *p;
+    size_t n;
+};
+__global__ void k_fill_seg(const FillArgs *__restrict__ tab, SegGrid S) {
+    Blk b;
+    const int r = seg_locate(S, b);
+    const FillArgs A = tab[r];
+    for (size_t i = (size_t)b.x * blockDim.x + threadIdx.x; i < A.n; i += (size_t)b.nx * blockDim.x) A.p[i] = 0.0;
+}
+static int build_fill_rho(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size_t &lds) {
+    *(FillArgs *)args = FillArgs{c->rho, c->nmesh};
+    size_t nb = (c->nmesh + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    dims = make_int4((int)nb, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int launch_fill(const void *tab, const SegGrid &S, int total, size_t, int, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_seg, dim3(total), dim3(256), 0, s, (const FillArgs *)tab, S);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+template <class T>
+static int build_assign(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size_t &lds) {
+    *(AssignArgs *)args = assign_args(c);
+    dims = make_int4(c->brick.nblock_assign, 1, 1, 0);
+    lds = c->brick.lds_assign ? 8 * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2] : 0;
+    return MDF_OK;
+}
+template <class T>
+static int launch_assign(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
+    MDF_TRY(allow_lds(k_assign_seg<T>, lds));
+    hipLaunchKernelGGL(k_assign_seg<T>, dim3(total), dim3(kPMThreads), lds, s, (const AssignArgs *)tab, S);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+template <class T>
+static int build_gather(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    GatherOut o{};
+    o.acc = io.acc;
+    o.af = io.af;
+    o.include_pp = 1;
+    o.vel = io.vel;
+    o.kick_dt = io.dt;
+    *(GatherArgs *)args = gather_args(c, o);
+    dims = make_int4(c->brick.nblock_gather, 1, 1, 0);
+    lds = c->brick.lds_gather ? 3 * sizeof(T) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2] : 0;
+    return MDF_OK;
+}
+template <class T>
+static int launch_gather(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
+    MDF_TRY(allow_lds(k_gather_seg<T>, lds));
+    hipLaunchKernelGGL(k_gather_seg<T>, dim3(total), dim3(kGatherThreads), lds, s, (const GatherArgs *)tab, S);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+void seg_stages_pm(std::vector<SegStage> &out, int precision, int part) {
+    const bool mixed = precision == MDF_PREC_MIXED;
+    if (part == 0) {
+        out.push_back({"fill_rho", sizeof(FillArgs), build_fill_rho, launch_fill});
+        if (mixed) out.push_back({"assign", sizeof(AssignArgs), build_assign<float>, launch_assign<float>});
+        else out.push_back({"assign", sizeof(AssignArgs), build_assign<double>, launch_assign<double>});
+    } else {
+        if (mixed) out.push_back({"gather", sizeof(GatherArgs), build_gather<float>, launch_gather<float>});
+        else out.push_back({"gather", sizeof(GatherArgs), build_gather<double>, launch_gather<double>});
+    }
 }
 
 }  // namespace mdf

@@ -161,11 +161,10 @@ __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, doubl
 __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 template <class R>
-__global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
+__device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsigned char *ldsraw) {
     using R2 = typename Vec2<R>::t;
     constexpr int D = ChebDeg<R>::v;
     constexpr bool kRel = sizeof(R) == 4;  // tile relative to the home block centre
-    extern __shared__ __align__(16) unsigned char ldsraw[];
     const int T = P.tile_max;
     R *tx = (R *)ldsraw;
     R *ty = tx + T;
@@ -183,7 +182,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
     const Geom &g = P.g;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int grp = lane / kGL, gl = lane % kGL;
-    const int bid = blockIdx.x;
+    const int bid = blk.x;
     const int bx = bid % g.nblk[0];
     const int by = (bid / g.nblk[0]) % g.nblk[1];
     const int bz = bid / (g.nblk[0] * g.nblk[1]);
@@ -349,6 +348,7 @@ __global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
                 pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
             };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
+            int qmine = 0;         // queue fill of this lane's half
             for (int rr = 0; rr < 25; ++rr) {
                 int2 lr = rng[h][rr];
                 if (!have_i) lr = make_int2(0, 0);
@@ -357,33 +357,38 @@ __global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
                 const int nit = (max(readlane_i(len, 0), readlane_i(len, kGL)) + kGL - 1) / kGL;
                 for (int it = 0; it < nit; ++it) {
                     const int j = lo + it * kGL + gl;
-                    bool hit = false;
-                    if (j < hi) {
-                        const R ddx = xi - tx[j], ddy = yi - ty[j], ddz = zi - tz[j];
-                        const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        hit = kRel ? ((r2 < rc2m) && (j != islot)) : ((r2 < rc2m) && (r2 > (R)0));
-                    }
+                    const bool inrow = j < hi;
+                    const int jc = inrow ? j : 0;  // slot 0 is always inside the LDS tile
+                    const R ddx = xi - tx[jc], ddy = yi - ty[jc], ddz = zi - tz[jc];
+                    const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    const bool hit =
+                        inrow && (r2 < rc2m) && (kRel ? (j != islot) : (r2 > (R)0));
                     const unsigned long long m = __ballot(hit);
-                    const unsigned mh = grp ? (unsigned)(m >> 32) : (unsigned)m;
-                    const int qn = grp ? qn1 : qn0;
-                    if (hit) q[qn + __popc(mh & ((1u << gl) - 1u))] = j;
-                    qn0 += __popc((unsigned)m);
-                    qn1 += __popc((unsigned)(m >> 32));
+                    const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
+                    const int clo = __popc(mlo), chi = __popc(mhi);
+                    // rank among this half's hits: bits below the lane, minus the lower half for upper lanes
+                    int pos = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+                    if (grp) pos -= clo;
+                    if (hit) q[qmine + pos] = j;
+                    qmine += grp ? chi : clo;
+                    qn0 += clo;
+                    qn1 += chi;
                     if (qn0 >= kGL || qn1 >= kGL) {
-                        const int cnt = min(grp ? qn1 : qn0, kGL);
+                        const int cnt = min(qmine, kGL);
                         if (gl < cnt) {
                             const int jj = q[gl];
                             pair_at(jj);
                         }
-                        const int rest = (grp ? qn1 : qn0) - cnt;
+                        const int rest = qmine - cnt;
                         const int moved = gl < rest ? q[kGL + gl] : 0;
                         if (gl < rest) q[gl] = moved;
                         qn0 = max(qn0 - kGL, 0);
                         qn1 = max(qn1 - kGL, 0);
+                        qmine = max(qmine - kGL, 0);
                     }
                 }
             }
-            if (gl < (grp ? qn1 : qn0)) {
+            if (gl < qmine) {
                 const int jj = q[gl];
                 pair_at(jj);
             }
@@ -423,6 +428,19 @@ __global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
     }
 }
 
+template <class R>
+__global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
+    extern __shared__ __align__(16) unsigned char ldsraw[];
+    pp_body<R>(P, blk_plain(), ldsraw);
+}
+template <class R>
+__global__ __launch_bounds__(kPPThreads) void k_pp_seg(const PPArgs<R> *__restrict__ tab, SegGrid S) {
+    extern __shared__ __align__(16) unsigned char ldsraw[];
+    Blk b;
+    const int r = seg_locate(S, b);
+    pp_body<R>(tab[r], b, ldsraw);
+}
+
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed) {
     // tile: (x, y, z) in FP64, or (x, y, z) hi and lo in FP32: 24 bytes a particle
     const size_t es = mixed ? 4 : 8;
@@ -431,7 +449,7 @@ size_t pp_lds_bytes(int tile_max, int npiece, bool mixed) {
 }
 
 template <class R>
-static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_t s) {
+static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb) {
     PPArgs<R> P;
     P.g = c->g;
     P.spos = c->spos;
@@ -447,6 +465,12 @@ static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_
     const double rcm = c->g.rc * (1.0 + 1e-12);
     P.rc2 = (R)(c->g.rc * c->g.rc);
     P.rc2m = sizeof(R) == 4 ? P.rc2 : (R)(rcm * rcm);
+    return P;
+}
+
+template <class R>
+static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_t s) {
+    const PPArgs<R> P = pp_args<R>(c, cheb);
     MDF_TRY(allow_lds(k_pp<R>, c->pp_lds));
     hipLaunchKernelGGL(k_pp<R>, dim3(c->nblock), dim3(kPPThreads), c->pp_lds, s, P);
     MDF_LAUNCHED();
@@ -457,6 +481,34 @@ int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s) {
     (void)n;
     if (c->prm.precision == MDF_PREC_MIXED) return launch_pp<float>(c, c->chebf, s);
     return launch_pp<double>(c, c->cheb, s);
+}
+
+// ---- segmented stage (replica batches) ----
+template <class R>
+static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size_t &lds) {
+    if ((sizeof(R) == 4) != (c->prm.precision == MDF_PREC_MIXED)) {
+        set_error("replica batch: mixed precisions");
+        return MDF_EINVAL;
+    }
+    *(PPArgs<R> *)args = pp_args<R>(c, sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->chebf
+                                                    : (const typename Vec2<R>::t *)c->cheb);
+    dims = make_int4(c->nblock, 1, 1, 0);
+    lds = c->pp_lds;
+    return MDF_OK;
+}
+template <class R>
+static int launch_pp_seg(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
+    MDF_TRY(allow_lds(k_pp_seg<R>, lds));
+    hipLaunchKernelGGL(k_pp_seg<R>, dim3(total), dim3(kPPThreads), lds, s, (const PPArgs<R> *)tab, S);
+    MDF_LAUNCHED();
+    return MDF_OK;
+}
+
+void seg_stages_pp(std::vector<SegStage> &out, int precision) {
+    if (precision == MDF_PREC_MIXED)
+        out.push_back({"pp", sizeof(PPArgs<float>), build_pp<float>, launch_pp_seg<float>});
+    else
+        out.push_back({"pp", sizeof(PPArgs<double>), build_pp<double>, launch_pp_seg<double>});
 }
 
 }  // namespace mdf

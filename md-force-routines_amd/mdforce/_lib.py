@@ -64,6 +64,8 @@ SIGNATURES = {
     "mdf_batch_create": [_i, ct.POINTER(P3MParams), _i, ct.POINTER(_vp)],
     "mdf_batch_destroy": [_vp],
     "mdf_batch_context": [_vp, _i, ct.POINTER(_vp)],
+    "mdf_batch_set_mode": [_vp, _i],
+    "mdf_batch_mode": [_vp],
     "mdf_batch_run": [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _vp, _vp, _i, _vp],
     "mdf_mlp_nparam": [],
     "mdf_mlp_create": [_i, _i, ct.POINTER(MLPMeta), ct.POINTER(_vp)],

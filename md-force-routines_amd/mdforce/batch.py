@@ -72,6 +72,14 @@ class ReplicaBatch:
             self.replicas.append(rep)
         self._keep = None
 
+    def set_mode(self, mode):
+        """0: segmented launches (one per stage for all replicas); 1: per-replica streams."""
+        check(self._lib.mdf_batch_set_mode(self._b, int(mode)), "mdf_batch_set_mode")
+
+    @property
+    def graph_mode(self):
+        return self._lib.mdf_batch_mode(self._b)
+
     def __len__(self):
         return len(self.systems)
 

@@ -32,7 +32,8 @@ def mods():
     return batch, engine
 
 
-def test_batch_graph_matches_single_replicas(mods):
+@pytest.mark.parametrize("mode", [0, 1], ids=["segmented", "streams"])
+def test_batch_graph_matches_single_replicas(mods, mode):
     batch, engine = mods
     specs = [(2100, 32, 2.0, 0.8, 4.0, 100.0), (5000, 32, 0.5, 0.6, 5.0, 10.0), (3300, 64, 3.0, 0.9, 3.9, 175.0)]
     systems, states = [], []
@@ -56,6 +57,7 @@ def test_batch_graph_matches_single_replicas(mods):
         e.close()
     # batch (2 graph launches of 4 steps + 2 eager steps)
     b = batch.ReplicaBatch(systems, n_streams=2)
+    b.set_mode(mode)
     P, V, A = [], [], []
     for rep, (pos, vel) in zip(b.replicas, states):
         p = torch.from_numpy(pos).cuda()
@@ -66,9 +68,10 @@ def test_batch_graph_matches_single_replicas(mods):
         A.append(a.clone())
     ens, nqts = b.run(P, V, A, nsteps, dt, nq=nq, graph_steps=K)
     torch.cuda.synchronize()
+    assert b.graph_mode == mode
     for r in range(len(systems)):
         p, v, a, en, nqt = singles[r]
-        print(f"[batch] replica {r}: max|dx| {np.abs(P[r].cpu().numpy() - p).max():.3e}")
+        print(f"[batch] mode {mode} replica {r}: max|dx| {np.abs(P[r].cpu().numpy() - p).max():.3e}")
         L = systems[r]["L"]
         dpos = np.abs(P[r].cpu().numpy() - p)
         assert np.minimum(dpos, L - dpos).max() <= 1e-11 * L
