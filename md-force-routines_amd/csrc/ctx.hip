@@ -206,10 +206,18 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             return rr;
         }
     }
-    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed);
-    if (c->pp_lds > 160 * 1024) {
-        c->tile_max -= (int)((c->pp_lds - 160 * 1024) / 24 + 4) & ~3;
-        c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed);
+    c->pp_waves = pp_lds_bytes(c->tile_max, c->npiece, mixed, 4) > 64 * 1024 ? 16 : 4;
+    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
+    // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB
+    const size_t lds_cap = 160 * 1024 - 4096;
+    if (c->pp_lds > lds_cap) {
+        c->tile_max -= (int)(((c->pp_lds - lds_cap) / 24 + 4) & ~(size_t)3);
+        c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
+    }
+    if (c->tile_max < 256) {
+        delete c;
+        set_error("PP pair table too large for LDS (kappa=%g alpha=%g rc=%g)", p->kappa, p->alpha, p->rc);
+        return MDF_EINVAL;
     }
 
     // PM blocks: the largest cb (<= 4 reference cells per axis) whose brick

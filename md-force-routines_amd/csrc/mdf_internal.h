@@ -65,8 +65,6 @@ struct Brick {
     int lds_gather;    // 1 if the 3-field brick fits LDS for the gather
 };
 
-constexpr int kPPThreads = 256;     // 4 waves share one 6x6x6 sub-cell tile
-constexpr int kPPWaves = kPPThreads / 64;
 constexpr int kPMThreads = 512;
 constexpr int kGatherThreads = 1024;
 constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces (FP64)
@@ -87,6 +85,7 @@ struct mdf_p3m_ctx {
     int nblock;         // total PP blocks
     int tile_max;       // LDS tile capacity (particles)
     size_t pp_lds;      // dynamic LDS bytes of the PP kernel
+    int pp_waves;       // waves per PP workgroup (4, or 16 for tiles above 64 KB)
     size_t nmesh;       // Mx*My*Mz
     size_t nspec;       // Mz*My*Mxh
     bool have_green;
@@ -174,7 +173,7 @@ int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
 // PP pair-term tables (host) and the PP kernel's dynamic LDS size.
 int build_pair_table(double kappa, double alpha, double rc, int deg, double tol, bool as_float,
                      std::vector<double2> &tab, int &npiece);
-size_t pp_lds_bytes(int tile_max, int npiece, bool mixed);
+size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw);
 // FP32 copy of the FP64 half-spectrum Green function (mixed mode).
 int green_to_float(mdf_p3m_ctx *c, hipStream_t s);
 // Learned correction of the attached network, added to acc_pp_s.
@@ -204,6 +203,7 @@ inline int allow_lds(F *kernel, size_t bytes) {
     if (bytes <= 65536) return MDF_OK;
     hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) {
+        (void)hipGetLastError();  // do not leave the error to the next launch check
         set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize=%zu): %s", bytes, hipGetErrorString(e));
         return MDF_EHIP;
     }

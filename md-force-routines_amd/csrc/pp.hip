@@ -160,7 +160,10 @@ __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, doubl
 
 __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-template <class R>
+// NW waves per workgroup share the tile: 4 for tiles up to 64 KB, 16 for
+// larger ones (large rc), where one workgroup per CU would otherwise leave a
+// single wave per SIMD.
+template <class R, int NW>
 __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsigned char *ldsraw) {
     using R2 = typename Vec2<R>::t;
     constexpr int D = ChebDeg<R>::v;
@@ -174,10 +177,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     R *lz = ly + (kRel ? T : 0);
     R2 *tab = (R2 *)(lz + (kRel ? T : 0));                 // npiece * (D + 1)
     int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
-    int *queue = cum + 224;                                // kPPWaves * kGroups * kQCap
+    int *queue = cum + 224;                                // NW * kGroups * kQCap
     __shared__ int tcell[216];
     __shared__ int2 rng[8][25];  // per home sub-cell: LDS range of each of its 25 stencil rows
-    __shared__ double red[kPPWaves];
+    __shared__ double red[NW];
 
     const Geom &g = P.g;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -257,7 +260,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
         }
         if (pass == 1 && tid == 0) atomicAdd(&P.stats[0], 1);
         // 3. stage rows [r0, r1) with their image shifts
-        for (int r = r0 + wave; r < r1; r += kPPWaves) {
+        for (int r = r0 + wave; r < r1; r += NW) {
             const int tys = r % 6, tzs = r / 6;
             int cy, cz;
             double sy, sz;
@@ -313,7 +316,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
         __syncthreads();
 
         // 4. two i per wave, one per half
-        for (int kb = wave * kGroups; kb < n_home; kb += kPPWaves * kGroups) {
+        for (int kb = wave * kGroups; kb < n_home; kb += NW * kGroups) {
             const int k = kb + grp;
             const bool have_i = k < n_home;
             const int kk = have_i ? k : kb;  // a valid home particle for the idle half
@@ -423,29 +426,29 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     __syncthreads();
     if (tid == 0) {
         double s = 0.0;
-        for (int w = 0; w < kPPWaves; ++w) s += red[w];
+        for (int w = 0; w < NW; ++w) s += red[w];
         P.part[bid] = 0.5 * s;
     }
 }
 
-template <class R>
-__global__ __launch_bounds__(kPPThreads) void k_pp(PPArgs<R> P) {
+template <class R, int NW>
+__global__ __launch_bounds__(NW * 64) void k_pp(PPArgs<R> P) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
-    pp_body<R>(P, blk_plain(), ldsraw);
+    pp_body<R, NW>(P, blk_plain(), ldsraw);
 }
-template <class R>
-__global__ __launch_bounds__(kPPThreads) void k_pp_seg(const PPArgs<R> *__restrict__ tab, SegGrid S) {
+template <class R, int NW>
+__global__ __launch_bounds__(NW * 64) void k_pp_seg(const PPArgs<R> *__restrict__ tab, SegGrid S) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
     Blk b;
     const int r = seg_locate(S, b);
-    pp_body<R>(tab[r], b, ldsraw);
+    pp_body<R, NW>(tab[r], b, ldsraw);
 }
 
-size_t pp_lds_bytes(int tile_max, int npiece, bool mixed) {
+size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
     // tile: (x, y, z) in FP64, or (x, y, z) hi and lo in FP32: 24 bytes a particle
     const size_t es = mixed ? 4 : 8;
     const size_t ncoef = (size_t)npiece * ((mixed ? kChebDegF : kChebDeg) + 1);
-    return 24 * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + kPPWaves * kGroups * kQCap);
+    return 24 * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + nw * kGroups * kQCap);
 }
 
 template <class R>
@@ -471,8 +474,13 @@ static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb) {
 template <class R>
 static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_t s) {
     const PPArgs<R> P = pp_args<R>(c, cheb);
-    MDF_TRY(allow_lds(k_pp<R>, c->pp_lds));
-    hipLaunchKernelGGL(k_pp<R>, dim3(c->nblock), dim3(kPPThreads), c->pp_lds, s, P);
+    if (c->pp_waves == 16) {
+        MDF_TRY(allow_lds(k_pp<R, 16>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 16>), dim3(c->nblock), dim3(16 * 64), c->pp_lds, s, P);
+    } else {
+        MDF_TRY(allow_lds(k_pp<R, 4>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 4>), dim3(c->nblock), dim3(4 * 64), c->pp_lds, s, P);
+    }
     MDF_LAUNCHED();
     return MDF_OK;
 }
@@ -496,10 +504,18 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
     lds = c->pp_lds;
     return MDF_OK;
 }
+// A launch's LDS is the largest of its replicas' (each computed for that
+// replica's wave count).  Above 64 KB the group holds a 16-wave replica, whose
+// reservation also covers the 16-wave queue of any 4-wave replica beside it.
 template <class R>
 static int launch_pp_seg(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
-    MDF_TRY(allow_lds(k_pp_seg<R>, lds));
-    hipLaunchKernelGGL(k_pp_seg<R>, dim3(total), dim3(kPPThreads), lds, s, (const PPArgs<R> *)tab, S);
+    if (lds > 65536) {
+        MDF_TRY(allow_lds(k_pp_seg<R, 16>, lds));
+        hipLaunchKernelGGL((k_pp_seg<R, 16>), dim3(total), dim3(16 * 64), lds, s, (const PPArgs<R> *)tab, S);
+    } else {
+        MDF_TRY(allow_lds(k_pp_seg<R, 4>, lds));
+        hipLaunchKernelGGL((k_pp_seg<R, 4>), dim3(total), dim3(4 * 64), lds, s, (const PPArgs<R> *)tab, S);
+    }
     MDF_LAUNCHED();
     return MDF_OK;
 }

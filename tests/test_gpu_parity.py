@@ -336,3 +336,33 @@ def test_n1e6_parity_vs_c_oracle(P3M):
     assert abs(en[0] - U) <= TOL * abs(U)
     assert e.stats()["pp_multipass_tiles"] == 0
     e.close()
+
+
+def test_large_rc_parity_vs_c_oracle(P3M):
+    """rc = 10 with ~330 particles per cell: the PP tile exceeds 64 KB (16-wave
+    workgroups) and the LDS capacity (several passes over the tile rows);
+    plain FP64 against the C restatement, and the same force in the mixed mode
+    to its FP32 bar."""
+    N, M = 20000, 32
+    L = O.box_length(N)
+    kappa, alpha, rc = 0.5, 0.45, 10.0
+    pos, _ = O.initial_split(21, 22, N, [L] * 3, 0.02)
+    e = P3M(N, M, L, kappa, alpha, rc)
+    G_full, _ = e.build_green(6, 3, want_full=True)
+    acc, en = e.force(dev(pos), None)
+    torch.cuda.synchronize()
+    U, acc_ref, app, apm, (upp, upm, uself) = c_oracle.force_pot(pos, np.ones(N), L, kappa, alpha, rc,
+                                                                  G_full.cpu().numpy(), M)
+    en = en.cpu().numpy()
+    st = e.stats()
+    print("large rc rel rms", rel_rms(acc.cpu().numpy(), acc_ref), "U", en[0], U, st)
+    assert rel_rms(acc.cpu().numpy(), acc_ref) <= TOL
+    assert abs(en[0] - U) <= TOL * abs(U)
+    assert st["pp_multipass_tiles"] > 0
+    e.close()
+    em = P3M(N, M, L, kappa, alpha, rc, precision="mixed")
+    em.build_green(6, 3)
+    accm, enm = em.force(dev(pos), None)
+    torch.cuda.synchronize()
+    assert rel_rms(accm.cpu().numpy(), acc_ref) <= 2e-6
+    em.close()
