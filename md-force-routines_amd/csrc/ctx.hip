@@ -206,7 +206,10 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             return rr;
         }
     }
-    c->pp_waves = pp_lds_bytes(c->tile_max, c->npiece, mixed, 4) > 64 * 1024 ? 16 : 4;
+    // waves per PP workgroup: 16 for tiles above 64 KB (one workgroup per CU);
+    // otherwise as many as the registers allow beside 4 LDS-bound workgroups
+    // per CU: 8 for the FP32 kernel (76 VGPR), 4 for FP64 (91 VGPR)
+    c->pp_waves = pp_lds_bytes(c->tile_max, c->npiece, mixed, 4) > 64 * 1024 ? 16 : (mixed ? 8 : 4);
     c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
     // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB
     const size_t lds_cap = 160 * 1024 - 4096;

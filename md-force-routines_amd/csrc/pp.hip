@@ -477,6 +477,9 @@ static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_
     if (c->pp_waves == 16) {
         MDF_TRY(allow_lds(k_pp<R, 16>, c->pp_lds));
         hipLaunchKernelGGL((k_pp<R, 16>), dim3(c->nblock), dim3(16 * 64), c->pp_lds, s, P);
+    } else if (c->pp_waves == 8) {
+        MDF_TRY(allow_lds(k_pp<R, 8>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 8>), dim3(c->nblock), dim3(8 * 64), c->pp_lds, s, P);
     } else {
         MDF_TRY(allow_lds(k_pp<R, 4>, c->pp_lds));
         hipLaunchKernelGGL((k_pp<R, 4>), dim3(c->nblock), dim3(4 * 64), c->pp_lds, s, P);
@@ -506,12 +509,16 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
 }
 // A launch's LDS is the largest of its replicas' (each computed for that
 // replica's wave count).  Above 64 KB the group holds a 16-wave replica, whose
-// reservation also covers the 16-wave queue of any 4-wave replica beside it.
+// reservation also covers the 16-wave queue of any 4- or 8-wave replica beside
+// it (those have at most 64 KB + 6 KB with a 16-wave queue).
 template <class R>
 static int launch_pp_seg(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
     if (lds > 65536) {
         MDF_TRY(allow_lds(k_pp_seg<R, 16>, lds));
         hipLaunchKernelGGL((k_pp_seg<R, 16>), dim3(total), dim3(16 * 64), lds, s, (const PPArgs<R> *)tab, S);
+    } else if (sizeof(R) == 4) {
+        MDF_TRY(allow_lds(k_pp_seg<R, 8>, lds));
+        hipLaunchKernelGGL((k_pp_seg<R, 8>), dim3(total), dim3(8 * 64), lds, s, (const PPArgs<R> *)tab, S);
     } else {
         MDF_TRY(allow_lds(k_pp_seg<R, 4>, lds));
         hipLaunchKernelGGL((k_pp_seg<R, 4>), dim3(total), dim3(4 * 64), lds, s, (const PPArgs<R> *)tab, S);
