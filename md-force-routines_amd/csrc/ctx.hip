@@ -193,7 +193,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     // larger tiles take more than one pass
     const double mean_tile = 216.0 * (double)p->n_max / (double)c->ncell;
     double tm = mean_tile + 6.0 * sqrt(mean_tile) + 64.0;
-    const double tcap = (150.0 * 1024.0 - 5120.0) / 24.0;
+    const double tcap = (150.0 * 1024.0 - 5120.0) / (mixed ? 32.0 : 24.0);
     if (tm > tcap) tm = tcap;
     if (tm < 512.0) tm = 512.0;
     c->tile_max = ((int)tm + 3) & ~3;  // keeps the table after the tile 16-byte aligned
@@ -214,7 +214,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB
     const size_t lds_cap = 160 * 1024 - 4096;
     if (c->pp_lds > lds_cap) {
-        c->tile_max -= (int)(((c->pp_lds - lds_cap) / 24 + 4) & ~(size_t)3);
+        c->tile_max -= (int)(((c->pp_lds - lds_cap) / (mixed ? 32 : 24) + 4) & ~(size_t)3);
         c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
     }
     if (c->tile_max < 256) {

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_modes(const double *__restrict__ pos, i
 // sincos.  Threads keep the chunk's sums in registers; each wave reduces
 // them once at the end and writes part[((m-1) * nwaves + wave) * 6 + c],
 // the layout k_modes_final sums.
-constexpr int kModeChunk = 16;
+constexpr int kModeChunk = 24;
 
 struct ModesArgs {
     const double *pos;

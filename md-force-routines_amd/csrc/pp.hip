@@ -169,13 +169,14 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     constexpr int D = ChebDeg<R>::v;
     constexpr bool kRel = sizeof(R) == 4;  // tile relative to the home block centre
     const int T = P.tile_max;
+    // FP64: three coordinate arrays (24 B a particle).  FP32: hi and lo as
+    // two float4 per particle (32 B), one 16-byte LDS read each
     R *tx = (R *)ldsraw;
     R *ty = tx + T;
     R *tz = ty + T;
-    R *lx = tz + T;                                        // lo parts (mixed only)
-    R *ly = lx + (kRel ? T : 0);
-    R *lz = ly + (kRel ? T : 0);
-    R2 *tab = (R2 *)(lz + (kRel ? T : 0));                 // npiece * (D + 1)
+    float4 *th4 = (float4 *)ldsraw;                        // mixed: (x, y, z, 0) hi
+    float4 *tl4 = th4 + T;                                 // mixed: lo parts
+    R2 *tab = kRel ? (R2 *)(tl4 + T) : (R2 *)(tz + T);     // npiece * (D + 1)
     int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // NW * kGroups * kQCap
     __shared__ int tcell[216];
@@ -287,13 +288,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                     const double4 p = P.spos[gs + t];
                     if (kRel) {
                         const double cx = (p.x + sx) - ox, cy = (p.y + sy) - oy, cz = (p.z + sz) - oz;
-                        const R hx = (R)cx, hy = (R)cy, hz = (R)cz;
-                        tx[dst + t] = hx;
-                        ty[dst + t] = hy;
-                        tz[dst + t] = hz;
-                        lx[dst + t] = (R)(cx - (double)hx);
-                        ly[dst + t] = (R)(cy - (double)hy);
-                        lz[dst + t] = (R)(cz - (double)hz);
+                        const float hx = (float)cx, hy = (float)cy, hz = (float)cz;
+                        th4[dst + t] = make_float4(hx, hy, hz, 0.0f);
+                        tl4[dst + t] = make_float4((float)(cx - (double)hx), (float)(cy - (double)hy),
+                                                   (float)(cz - (double)hz), 0.0f);
                     } else {
                         tx[dst + t] = (R)(p.x + sx);
                         ty[dst + t] = (R)(p.y + sy);
@@ -342,11 +340,16 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             const int islot = (irow >= r0 && irow < r1) ? cum[hidx] - base + (kk - hb) : -1;
             R fx = 0, fy = 0, fz = 0, u = 0;
             auto pair_at = [&](int jj) {
-                R dx = xi - tx[jj], dy = yi - ty[jj], dz = zi - tz[jj];
-                if (kRel) {
-                    dx += xil - lx[jj];
-                    dy += yil - ly[jj];
-                    dz += zil - lz[jj];
+                R dx, dy, dz;
+                if constexpr (kRel) {
+                    const float4 a = th4[jj], b = tl4[jj];
+                    dx = (xi - a.x) + (xil - b.x);
+                    dy = (yi - a.y) + (yil - b.y);
+                    dz = (zi - a.z) + (zil - b.z);
+                } else {
+                    dx = xi - tx[jj];
+                    dy = yi - ty[jj];
+                    dz = zi - tz[jj];
                 }
                 pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
             };
@@ -362,7 +365,17 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                     const int j = lo + it * kGL + gl;
                     const bool inrow = j < hi;
                     const int jc = inrow ? j : 0;  // slot 0 is always inside the LDS tile
-                    const R ddx = xi - tx[jc], ddy = yi - ty[jc], ddz = zi - tz[jc];
+                    R ddx, ddy, ddz;
+                    if constexpr (kRel) {
+                        const float4 a = th4[jc];
+                        ddx = xi - a.x;
+                        ddy = yi - a.y;
+                        ddz = zi - a.z;
+                    } else {
+                        ddx = xi - tx[jc];
+                        ddy = yi - ty[jc];
+                        ddz = zi - tz[jc];
+                    }
                     const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     const bool hit =
                         inrow && (r2 < rc2m) && (kRel ? (j != islot) : (r2 > (R)0));
@@ -445,10 +458,10 @@ __global__ __launch_bounds__(NW * 64) void k_pp_seg(const PPArgs<R> *__restrict_
 }
 
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
-    // tile: (x, y, z) in FP64, or (x, y, z) hi and lo in FP32: 24 bytes a particle
+    // tile: (x, y, z) in FP64 (24 B a particle), or hi and lo float4 in FP32 (32 B)
     const size_t es = mixed ? 4 : 8;
     const size_t ncoef = (size_t)npiece * ((mixed ? kChebDegF : kChebDeg) + 1);
-    return 24 * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + nw * kGroups * kQCap);
+    return (mixed ? 32 : 24) * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + nw * kGroups * kQCap);
 }
 
 template <class R>
