@@ -96,6 +96,18 @@ def pmc_traffic(stage, form, N, M, rc, alpha):
     return None
 
 
+def pp_waves(N, L, rc, mixed):
+    """Waves per PP workgroup as mdf_p3m_create picks them: 16 when the LDS
+    tile (mean + 6 sigma + 64 particles of a 6^3 sub-cell tile, 32 B each in
+    FP32, 24 B in FP64) exceeds 64 KB, else 8 (FP32) or 4 (FP64)."""
+    nsub = 2 * int(L // rc)
+    mean = 216.0 * N / nsub ** 3
+    tile = mean + 6.0 * math.sqrt(mean) + 64.0
+    if tile * (32 if mixed else 24) > 64 * 1024:
+        return 16
+    return 8 if mixed else 4
+
+
 def stage_work(stage, N, M, rc):
     """Algorithmic work of one launch of a stage: (bytes, flops).
 
@@ -438,7 +450,7 @@ def main():
         roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = pmc_traffic(dom, main_form, N, M, a.rc, a.alpha)
-    pp_name = "k_pp<float>" if main_form == "mixed" else "k_pp<double>"
+    pp_name = f"k_pp<{'float' if main_form == 'mixed' else 'double'}, {pp_waves(N, L, a.rc, main_form == 'mixed')}>"
     roof["kernel"] = {"pp": pp_name, "fft": "fft passes", "gather": "k_gather", "assign": "k_assign",
                       "bin": "bin kernels", "drift": "k_kick_drift", "modes": "k_modes_grid"}.get(dom, dom)
     roof["avg_launch_ms"] = avg_s * 1e3
