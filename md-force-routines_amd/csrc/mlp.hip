@@ -110,21 +110,26 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Weights in LDS (BF16 bits).
-struct alignas(16) MLPShared {
+// Weights and activation tiles in LDS (BF16 bits).  The forward kernel uses
+// the first part only (39 KB, 4 workgroups per CU); the backward kernel adds
+// the row-major weights and the dz tile (63 KB).
+struct alignas(16) MLPFwdShared {
     unsigned short w1t[kH][32];   // W1^T, K padded to 32 with zeros
     unsigned short w2t[kH][kH];   // W2^T
     unsigned short w3t[kH][kH];   // W3^T
-    unsigned short w2[kH][kH];    // W2 (backward)
-    unsigned short w3[kH][kH];    // W3 (backward)
     float b1[kH], b2[kH], b3[kH], w4[kH];
     unsigned short xb[kMLPWaves][16][kIn];   // per-wave input tile
     unsigned short h1[kMLPWaves][16][kH];
     unsigned short h2[kMLPWaves][16][kH];
+};
+struct alignas(16) MLPShared : MLPFwdShared {
+    unsigned short w2[kH][kH];    // W2 (backward)
+    unsigned short w3[kH][kH];    // W3 (backward)
     unsigned short d[kMLPWaves][16][kH];     // dz tile (backward)
 };
 
-__device__ void stage_weights(MLPShared &S, const float *__restrict__ P, bool backward) {
+template <class SH>
+__device__ void stage_weights(SH &S, const float *__restrict__ P, bool backward) {
     for (int t = threadIdx.x; t < kH * 32; t += blockDim.x) {
         const int n = t / 32, k = t % 32;
         S.w1t[n][k] = k < kIn ? bf16_bits(P[kOffW1 + k * kH + n]) : (unsigned short)0;
@@ -133,9 +138,11 @@ __device__ void stage_weights(MLPShared &S, const float *__restrict__ P, bool ba
         const int a = t / kH, b = t % kH;
         S.w2t[b][a] = bf16_bits(P[kOffW2 + t]);  // t = a*64 + b: W2[a][b]
         S.w3t[b][a] = bf16_bits(P[kOffW3 + t]);
-        if (backward) {
-            S.w2[a][b] = bf16_bits(P[kOffW2 + t]);
-            S.w3[a][b] = bf16_bits(P[kOffW3 + t]);
+        if constexpr (sizeof(SH) > sizeof(MLPFwdShared)) {
+            if (backward) {
+                S.w2[a][b] = bf16_bits(P[kOffW2 + t]);
+                S.w3[a][b] = bf16_bits(P[kOffW3 + t]);
+            }
         }
     }
     for (int t = threadIdx.x; t < kH; t += blockDim.x) {
@@ -222,7 +229,8 @@ struct Fwd {
     float out[4];    // network output of rows 4*grp + i (valid in every lane of the group)
 };
 
-__device__ __forceinline__ void forward_tile(const MLPArgs &P, MLPShared &S, int wave, int p0, int total, Fwd &F) {
+template <class SH>
+__device__ __forceinline__ void forward_tile(const MLPArgs &P, SH &S, int wave, int p0, int total, Fwd &F) {
     const int lane = threadIdx.x & 63, m = lane & 15, grp = lane >> 4;
     const int p = p0 + m;
     const bool valid = p < total;
@@ -285,7 +293,7 @@ __device__ __forceinline__ void forward_tile(const MLPArgs &P, MLPShared &S, int
 }
 
 __global__ __launch_bounds__(kMLPThreads) void k_mlp_forward(MLPArgs P) {
-    __shared__ MLPShared S;
+    __shared__ MLPFwdShared S;
     stage_weights(S, P.param, false);
     __syncthreads();
     const int total = min(P.off[P.n], P.max_pairs);

@@ -107,3 +107,51 @@ def test_autotuner_meets_budget(mods):
     e2.build_green()
     acc2, _ = e2.force(p, None, af=1.0)
     assert autotune.force_error(acc2, ref) <= 1e-7
+
+
+def test_batch_without_modes_and_mixed_precisions_fall_back(mods):
+    """nq = 0 drops the n(q) stages of the segmented graph; a batch whose
+    replicas differ in precision cannot be segmented and runs as per-replica
+    streams.  Both still match the replicas stepped alone."""
+    batch, engine = mods
+    specs = [(2500, 32, 2.0, 0.8, 4.0, 100.0, "fp64"), (3100, 32, 1.0, 0.7, 4.5, 50.0, "mixed")]
+    for uniform in (True, False):
+        systems, states = [], []
+        for r, (n, M, kappa, alpha, rc, gamma, prec) in enumerate(specs):
+            L, pos, vel = _system(n, 10 + r, gamma)
+            systems.append(dict(n=n, L=L, kappa=kappa, mesh=M, alpha=alpha, rc=rc,
+                                precision="fp64" if uniform else prec))
+            states.append((pos, vel))
+        singles = []
+        for s, (pos, vel) in zip(systems, states):
+            e = engine.P3M(s["n"], s["mesh"], s["L"], s["kappa"], s["alpha"], s["rc"], precision=s["precision"])
+            e.build_green()
+            p = torch.from_numpy(pos).cuda()
+            v = torch.from_numpy(vel).cuda()
+            a, _ = e.force(p)
+            a = a.clone()
+            en, _ = e.md_run(p, v, a, 6, 0.1)
+            torch.cuda.synchronize()
+            singles.append((p.cpu().numpy(), en.cpu().numpy()))
+            e.close()
+        b = batch.ReplicaBatch(systems, n_streams=2)
+        P, V, A = [], [], []
+        for rep, (pos, vel) in zip(b.replicas, states):
+            p = torch.from_numpy(pos).cuda()
+            v = torch.from_numpy(vel).cuda()
+            a, _ = rep.force(p)
+            P.append(p)
+            V.append(v)
+            A.append(a.clone())
+        ens, nqts = b.run(P, V, A, 6, 0.1, nq=0, graph_steps=3)
+        torch.cuda.synchronize()
+        print(f"[batch] uniform={uniform}: graph mode {b.graph_mode}")
+        assert nqts is None
+        assert b.graph_mode == (0 if uniform else 1)
+        for r, s in enumerate(systems):
+            p, en = singles[r]
+            tol = 1e-11 if s["precision"] == "fp64" else 1e-6
+            d = np.abs(P[r].cpu().numpy() - p)
+            assert np.minimum(d, s["L"] - d).max() <= tol * s["L"]
+            np.testing.assert_allclose(ens[r].cpu().numpy(), en, rtol=tol, atol=tol * np.abs(en).max())
+        b.close()
