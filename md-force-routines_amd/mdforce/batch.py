@@ -4,9 +4,11 @@ Many independent seeded systems - the runs whose n(q, t) the reference
 averages into S(q, w) (s_of_q_w_avg.py) - advance together: each replica has
 its own context (box, mesh, kappa, alpha, rc), and the NVE steps of all
 replicas are captured into one hipGraph of ``graph_steps`` steps that is
-replayed, with replicas overlapping on worker streams.  n(q, t) on the grid
-q_m = m * 2 pi / L_r (m = 1..nq, along x, y and z) and the per-step (E, K,
-U, T) are written to device arrays for every step.
+replayed.  By default each stage of the step is one segmented launch for all
+replicas (``set_mode(0)``); ``set_mode(1)`` captures each replica's own
+launches on worker streams instead.  n(q, t) on the grid q_m = m * 2 pi / L_r
+(m = 1..nq, along x, y and z) and the per-step (E, K, U, T) are written to
+device arrays for every step.
 """
 from __future__ import annotations
 

@@ -46,6 +46,10 @@ class P3M:
     (yukawa_p3m_ord6.py:6): N (as n_max), Lv, (Mx, My, Mz), kappa, G (alpha),
     rc.  The Green function is set from the reference's gf_opt array
     (``set_green``) or computed on the device (``build_green``).
+
+    precision: "fp64" (also "plain"; the reference's arithmetic) or "mixed"
+    (FP32 pair math, assignment weights and FFT passes; FP64 charge-mesh
+    accumulation, energies, reductions and positions).
     """
 
     def __init__(self, n_max, mesh, box, kappa, alpha, rc, precision="fp64", device=None):
