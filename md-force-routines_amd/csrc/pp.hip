@@ -24,12 +24,12 @@
 //   their periodic image shift applied, as 36 rows of 6 sub-cells (one or two
 //   contiguous runs of the sorted particle array each).
 // * Sweep.  A wave handles two i at a time, one per 32-lane half.  Each half
-//   sweeps the 5x5 stencil rows (5 sub-cells each, one contiguous LDS range
-//   per row, tabulated once per home sub-cell) of its i, testing 32
-//   candidates per step; in-cutoff j are compacted into the half's LDS queue
-//   with a ballot and popcount, and the pair terms run only on queue
-//   batches, so lanes stay busy despite the ~25% in-cutoff rate of a cell
-//   stencil.
+//   sweeps the 5 planes of its i's 5x5x5 sub-cell stencil, each plane one
+//   contiguous LDS range (its 5 rows are consecutive tile rows; tabulated
+//   once per home sub-cell), testing 32 candidates per step; in-cutoff j
+//   are compacted into the half's LDS queue with a ballot and mbcnt, and the
+//   pair terms run only on queue batches, so lanes stay busy despite the
+//   ~25% in-cutoff rate of a cell stencil.
 // * Each pair is evaluated from both ends (no Newton-3 scatter, no atomics);
 //   i's force and energy are summed across its half-wave with shuffles.
 //   The energy is halved at the end: U_s_r = 1/2 sum_i sum_{j != i} phi.
@@ -180,7 +180,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // NW * kGroups * kQCap
     __shared__ int tcell[216];
-    __shared__ int2 rng[8][25];  // per home sub-cell: LDS range of each of its 25 stencil rows
+    __shared__ int2 rng[8][5];   // per home sub-cell: LDS range of each of its 5 stencil planes
     __shared__ double red[NW];
 
     const Geom &g = P.g;
@@ -301,15 +301,21 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 txs = txe;
             }
         }
-        // row ranges of this pass: home sub-cell h, stencil row (dy, dz) covers
-        // tile x sub-cells hx .. hx+4 of tile row (2+hy+dy, 2+hz+dz)
-        if (tid < 200) {
-            const int h = tid / 25, rr = tid % 25;
+        // plane ranges of this pass: home sub-cell h needs tile x sub-cells
+        // hx .. hx+4 of the five tile rows (hy+dy, hz+dz), dy = 0..4, of each
+        // plane dz.  Those rows are consecutive in LDS, so one range from row
+        // hy's sub-cell hx to row hy+4's sub-cell hx+4 covers them; it also
+        // takes in one sub-cell three away along x between consecutive rows,
+        // whose particles are at least 2 sub-cell widths >= rc away and fail
+        // the cutoff test.  Rows outside this pass are left out.
+        if (tid < 40) {
+            const int h = tid / 5, pz = tid % 5;
             const int hx = h & 1, hy = (h >> 1) & 1, hz = h >> 2;
-            const int r = (hy + rr % 5) + 6 * (hz + rr / 5);
+            const int ra = max(hy + 6 * (hz + pz), r0);
+            const int rb = min(hy + 4 + 6 * (hz + pz), r1 - 1);
             int2 v = make_int2(0, 0);
-            if (r >= r0 && r < r1) v = make_int2(cum[6 * r + hx] - base, cum[6 * r + hx + 5] - base);
-            rng[h][rr] = v;
+            if (ra <= rb) v = make_int2(cum[6 * ra + hx] - base, cum[6 * rb + hx + 5] - base);
+            rng[h][pz] = v;
         }
         __syncthreads();
 
@@ -355,7 +361,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
             int qmine = 0;         // queue fill of this lane's half
-            for (int rr = 0; rr < 25; ++rr) {
+            for (int rr = 0; rr < 5; ++rr) {
                 int2 lr = rng[h][rr];
                 if (!have_i) lr = make_int2(0, 0);
                 const int lo = lr.x, hi = lr.y;
