@@ -334,16 +334,38 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             const int hx = h & 1, hy = (h >> 1) & 1, hz = h >> 2;
             const int hidx = (2 + hx) + 6 * ((2 + hy) + 6 * (2 + hz));
             const int gi = P.cell_start[tcell[hidx]] + (kk - hb);
-            const double4 pi = P.spos[gi];
-            // same rounding as the staging of the home rows (image shift 0)
-            const double cxi = (pi.x + 0.0) - ox, cyi = (pi.y + 0.0) - oy, czi = (pi.z + 0.0) - oz;
-            const R xi = (R)cxi, yi = (R)cyi, zi = (R)czi;
-            const R xil = kRel ? (R)(cxi - (double)xi) : (R)0;
-            const R yil = kRel ? (R)(cyi - (double)yi) : (R)0;
-            const R zil = kRel ? (R)(czi - (double)zi) : (R)0;
             // tile slot of i in this pass (its row may be outside the pass)
             const int irow = (2 + hy) + 6 * (2 + hz);
             const int islot = (irow >= r0 && irow < r1) ? cum[hidx] - base + (kk - hb) : -1;
+            // i's coordinates: from its tile slot when staged (home rows have image
+            // shift 0, so the values are those of the staging), else from memory
+            R xi, yi, zi, xil = 0, yil = 0, zil = 0;
+            if (islot >= 0) {
+                if constexpr (kRel) {
+                    const float4 a = th4[islot], b = tl4[islot];
+                    xi = a.x;
+                    yi = a.y;
+                    zi = a.z;
+                    xil = b.x;
+                    yil = b.y;
+                    zil = b.z;
+                } else {
+                    xi = tx[islot];
+                    yi = ty[islot];
+                    zi = tz[islot];
+                }
+            } else {
+                const double4 pi = P.spos[gi];
+                const double cxi = (pi.x + 0.0) - ox, cyi = (pi.y + 0.0) - oy, czi = (pi.z + 0.0) - oz;
+                xi = (R)cxi;
+                yi = (R)cyi;
+                zi = (R)czi;
+                if (kRel) {
+                    xil = (R)(cxi - (double)xi);
+                    yil = (R)(cyi - (double)yi);
+                    zil = (R)(czi - (double)zi);
+                }
+            }
             R fx = 0, fy = 0, fz = 0, u = 0;
             auto pair_at = [&](int jj) {
                 R dx, dy, dz;
