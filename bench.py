@@ -60,11 +60,16 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-err", action="store_true", help="skip the force-error measurement")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp"), default="n1e6",
+    ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "drift"), default="n1e6",
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--mlp-epochs", type=int, default=20)
+    ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--eq-steps", type=int, default=300)
+    ap.add_argument("--uniform-steps", type=int, default=1000)
+    ap.add_argument("--start", choices=("both", "uniform", "equilibrated"), default="both",
+                    help="drift workload: start(s) of the trajectories")
     ap.add_argument("--autotune", action="store_true",
                     help="alpha and rc from the autotuner (FP32-mixed, budget 1e-5, the given mesh)")
     ap.add_argument("--graph-steps", type=int, default=50, help="0 = no graph (eager launches)")
@@ -328,6 +333,109 @@ def run_mlp(a):
     print(json.dumps(out), flush=True)
 
 
+def run_drift(a):
+    """Energy conservation and force error along a trajectory, both forms.
+
+    Two starts of N = 1e6 (Gamma 100, kappa 2, 128^3, alpha/rc of the n1e6
+    workload):
+      uniform       NVE straight from the seeded uniform positions and
+                    Maxwellian velocities (the n1e6 workload's own start),
+                    --uniform-steps steps (1000: the release of the random
+                    start's close pairs heats this system without bound at
+                    dt 0.1, so its energy is not conserved by either form);
+      equilibrated  the same start after --eq-steps steps with a velocity
+                    rescale to T every 10 steps (thermostat_p3m_ord6.py:15-18),
+                    preceded by 100 steps at dt/10 and with each ion's speed
+                    clipped to 4 sqrt(3T) at every rescale, so that the release
+                    of the closest random pairs leaves no fast ions behind.
+      From the equilibrated start each form runs --drift-steps NVE steps.
+    Every 100 steps of every trajectory the force is compared with tight FP64
+    P3M (alpha 0.6, rc 9, 512^3) on the current positions.  Reported per start and form: max |E(t) - E(0)| /
+    (N T), the slope of a linear fit of (E - E(0)) / (N T) per 1000 steps,
+    the force error (RMS over mean |F|) every 100 steps, the largest speed at
+    the end, and ms per step."""
+    import numpy as np
+    import torch
+
+    from mdforce import mlp_train as T_
+    from mdforce.engine import P3M
+    from mdforce.initialize_pos_vel import initial_split
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N, M = a.n, a.mesh
+    L = (4.0 * math.pi * N / 3.0) ** (1.0 / 3.0)
+    T = 1.0 / a.gamma
+    pos0, vel0 = initial_split(1, 2, N, [L, L, L], T)
+    ref = P3M(N, 512, L, a.kappa, 0.6, 9.0, device=dev)
+    ref.build_green()
+    starts = {}
+    if a.start in ("uniform", "both"):
+        starts["uniform"] = (torch.from_numpy(pos0).to(dev), torch.from_numpy(vel0).to(dev))
+    if a.start in ("equilibrated", "both"):
+        pos = torch.from_numpy(pos0).to(dev)
+        vel = torch.from_numpy(vel0).to(dev)
+        eq = P3M(N, M, L, a.kappa, a.alpha, a.rc, device=dev)
+        eq.build_green()
+        T_.equilibrate(eq, pos, vel, T, a.eq_steps, a.dt, vcap=4.0, soft_steps=100)
+        eq.close()
+        starts["equilibrated"] = (pos, vel)
+    chunk = 100
+    out = {}
+    for sname, (pos, vel) in starts.items():
+        nchunk = max(1, (a.uniform_steps if sname == "uniform" else a.drift_steps) // chunk)
+        out[sname] = {}
+        for prec in ("fp64", "mixed"):
+            e = P3M(N, M, L, a.kappa, a.alpha, a.rc, precision=prec, device=dev)
+            e.build_green()
+            p, v = pos.clone(), vel.clone()
+            acc, _ = e.force(p)
+            acc = acc.clone()
+            en_all, errs = [], []
+            t_ms = 0.0
+            for c in range(nchunk):
+                fr, _ = ref.force(p)
+                d = acc - fr  # both af (pp + pm) with the default af
+                errs.append(float(torch.sqrt((d * d).sum(1).mean()) / torch.sqrt((fr * fr).sum(1)).mean()))
+                s0 = torch.cuda.Event(enable_timing=True)
+                s1 = torch.cuda.Event(enable_timing=True)
+                s0.record()
+                en, _ = e.md_run(p, v, acc, chunk, a.dt)
+                s1.record()
+                torch.cuda.synchronize()
+                t_ms += s0.elapsed_time(s1)
+                en_all.append(en[:, 0].cpu().numpy().copy())
+                if c % 10 == 0:
+                    print(f"[drift] {sname} {prec} step {(c + 1) * chunk}: dE/(N T) "
+                          f"{(en_all[-1][-1] - en_all[0][0]) / (N * T):.3e} T {float(en[-1, 3]):.5f} "
+                          f"vmax {float(v.norm(dim=1).max()):.3f} err {errs[-1]:.2e}", file=sys.stderr, flush=True)
+            E = np.concatenate(en_all)
+            de = (E - E[0]) / (N * T)
+            steps = np.arange(1, len(E) + 1)
+            slope = float(np.polyfit(steps, de, 1)[0] * 1000.0)
+            out[sname]["chosen" if prec == "mixed" else "plain"] = {
+                "ms_per_step": t_ms / (nchunk * chunk), "energy_drift_max_per_NT": float(np.abs(de).max()),
+                "energy_drift_end_per_NT": float(de[-1]), "energy_drift_slope_per_NT_per_1000_steps": slope,
+                "force_error_max": max(errs), "force_error_mean": float(np.mean(errs)),
+                "force_error_every_100_steps": [float(f"{x:.3e}") for x in errs],
+                "vmax_end": float(v.norm(dim=1).max()), "steps": nchunk * chunk}
+            e.close()
+    ref.close()
+    first = out.get("equilibrated", next(iter(out.values())))
+    res = {
+        "metric": METRIC + " (energy drift over a trajectory)",
+        "value": N / (first["chosen"]["ms_per_step"] / 1e3), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": first["chosen"]["steps"], "warmup": a.eq_steps, "ms_per_step": first["chosen"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"N={N}, {M}^3, Gamma {a.gamma:g}, kappa {a.kappa:g}, NVE trajectories of both "
+                               f"forms from " + ", ".join(f"the {k} start ({v['chosen']['steps']} steps)"
+                                                          for k, v in out.items()),
+                   "alpha": a.alpha, "rc": a.rc, "dt": a.dt, "starts": out, "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -339,6 +447,9 @@ def main():
         return
     if a.workload == "mlp":
         run_mlp(a)
+        return
+    if a.workload == "drift":
+        run_drift(a)
         return
     if a.workload == "n8e6":
         # large weakly screened system: N = 8e6, Gamma 50, kappa 0.5, 256^3, dt 0.05, autotuned alpha/rc

@@ -7,9 +7,14 @@
 //   1. count: sub-cell of each particle, slot = atomicAdd(count[cell])
 //   2. scan : exclusive prefix sum of the counts (and reset of the counts)
 //   3. place: sorted index = start[cell] + slot; sorted (x, y, z, Z) copy.
-// Positions are wrapped once into [0, L] for the sorted copy (a position at
+// Positions are wrapped into [0, L) for the sorted copy (a position at
 // exactly L, which the reference's cell index leaves out of range, lands in
 // the periodic image at 0); in-range positions are copied bit for bit.
+// Every kernel that turns a position into a cell or mesh index reads the
+// sorted copy, so the wrap must hold for any input: the Verlet drift wraps
+// once per step (velocity_verlet_p3m_ord6.py:15-22), and an ion faster than
+// L/dt leaves the box by more than one period.  Such positions are reduced
+// with floor(x / L); a non-finite one is placed at 0.
 #include "mdf_internal.h"
 
 namespace mdf {
@@ -17,6 +22,10 @@ namespace mdf {
 __device__ __forceinline__ double wrap_box(double x, double L) {
     if (x < 0.0) x += L;
     if (x >= L) x -= L;
+    if (!(x >= 0.0 && x < L)) {
+        x = __builtin_isfinite(x) ? x - L * floor(x / L) : 0.0;
+        if (!(x >= 0.0 && x < L)) x = 0.0;  // x - L floor(x/L) may round to L
+    }
     return x;
 }
 

@@ -134,7 +134,7 @@ __device__ __forceinline__ void pp_pair_cheb(const PPArgs<T> &P, const typename 
     const T ir = rsqrt_t<T>(r2);
     const T r = r2 * ir;
     int p = (int)(r * P.inv_piece_w);
-    p = p < P.npiece - 1 ? p : P.npiece - 1;
+    p = p < P.npiece - 1 ? (p > 0 ? p : 0) : P.npiece - 1;
     const T t = (r - p * P.piece_w) * ((T)2 * P.inv_piece_w) - (T)1;
     T g1, g2;
     cheb2<T>(tab + p * (D + 1), t, g1, g2);

@@ -31,15 +31,34 @@ def box_length(n):
     return (4.0 * math.pi * n / 3.0) ** (1.0 / 3.0)
 
 
-def equilibrate(eng, pos, vel, T, steps, dt=0.1, rescale_every=10):
-    """NVE steps with a velocity rescale to T every rescale_every steps (in place)."""
+def equilibrate(eng, pos, vel, T, steps, dt=0.1, rescale_every=10, vcap=None, soft_steps=0):
+    """NVE steps with a velocity rescale to T every rescale_every steps (in place).
+
+    A uniform random start holds a few pairs at distances ~N^(-1/3); at large
+    N their release at dt = 0.1 throws single ions out at tens of thermal
+    speeds, and a global rescale does not slow them down.  vcap (in units of
+    sqrt(3 T)) clips each ion's speed at every rescale, and the first
+    soft_steps steps run at dt / 10 with a rescale and clip after every step,
+    so that the release itself is integrated accurately and its energy
+    removed.  Both default off (the plain rescale protocol)."""
+    def clip():
+        if vcap is not None:
+            s = vel.norm(dim=1, keepdim=True)
+            vel.mul_(torch.clamp(vcap * math.sqrt(3.0 * T) / s.clamp_min(1e-300), max=1.0))
+
     acc, _ = eng.force(pos)
     acc = acc.clone()
+    for _ in range(soft_steps):
+        en, _ = eng.md_run(pos, vel, acc, 1, dt / 10.0)
+        clip()
+        ksum = (vel * vel).sum().reshape(1)
+        eng.thermostat(vel, ksum, T, 0)
     done = 0
     while done < steps:
         k = min(rescale_every, steps - done)
         en, _ = eng.md_run(pos, vel, acc, k, dt)
-        ksum = (en[-1, 1:2] / 1.5).contiguous()
+        clip()
+        ksum = (vel * vel).sum().reshape(1) if vcap is not None else (en[-1, 1:2] / 1.5).contiguous()
         eng.thermostat(vel, ksum, T, 0)
         done += k
     return acc

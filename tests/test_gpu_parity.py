@@ -293,6 +293,31 @@ def test_edge_positions_wrap_like_periodic_images(P3M):
     e.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_positions_several_periods_out(P3M, precision):
+    """Ions displaced by whole periods (up to +-5 L, one by 1000 L), as a
+    drift faster than L/dt leaves them: the force is that of the in-box
+    positions (the sorted copy reduces them with floor(x/L))."""
+    N = 2000
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 1.0, 0.8, L / 3.3, 32)
+    pos, _ = O.initial_split(23, 24, N, P.Lv, 0.01)
+    U_ref, acc_ref, _ = P.force(pos)
+    k = np.random.RandomState(5).randint(-5, 6, size=pos.shape)
+    k[0] = [1000, -1000, 7]
+    far = pos + k * L
+    print(f"[wrap] {precision}: {int((k != 0).any(1).sum())} of {N} ions outside the box, max |k| {np.abs(k).max()}")
+    e = P3M(N, 32, L, 1.0, 0.8, L / 3.3, precision=precision)
+    e.set_green(dev(P.G_k))
+    acc, en = e.force(dev(far))
+    torch.cuda.synchronize()
+    tol = 1e-9 if precision == "fp64" else 2e-6
+    assert np.isfinite(acc.cpu().numpy()).all()
+    assert rel_rms(acc.cpu().numpy(), acc_ref) <= tol
+    assert float(en[0]) == pytest.approx(U_ref, rel=tol)
+    e.close()
+
+
 def test_ragged_and_tiny_inputs(P3M):
     L = 30.0
     e = P3M(1000, 16, L, 1.0, 0.7, 9.0)
