@@ -9,7 +9,7 @@ n(q, t) for the 20 smallest q along each axis.  alpha = 0.40 and rc = 7.0
 hold the RMS force error to at most 1e-5 of the mean force on the
 equilibrated liquid (Gamma 100), where the mean force is ~20x smaller than on
 the uniform start and the budget binds.  The error against a tight FP64
-reference (alpha 0.6, rc 9, 512^3) is measured at setup on both states (the
+reference (autotune.reference_settings) is measured at setup on both states (the
 liquid: the same start after 100 steps at dt/10 and --eq-steps steps with a
 velocity rescale every 10 steps and a per-ion speed clip) and reported.
 
@@ -108,18 +108,6 @@ def pmc_traffic(stage, form, N, M, rc, alpha):
     return None
 
 
-def pp_waves(N, L, rc, mixed):
-    """Waves per PP workgroup as mdf_p3m_create picks them: 16 when the LDS
-    tile (mean + 6 sigma + 64 particles of a 6^3 sub-cell tile, 32 B each in
-    FP32, 24 B in FP64) exceeds 64 KB, else 8 (FP32) or 4 (FP64)."""
-    nsub = 2 * int(L // rc)
-    mean = 216.0 * N / nsub ** 3
-    tile = mean + 6.0 * math.sqrt(mean) + 64.0
-    if tile * (32 if mixed else 24) > 64 * 1024:
-        return 16
-    return 8 if mixed else 4
-
-
 def stage_work(stage, N, M, rc):
     """Algorithmic work of one launch of a stage: (bytes, flops).
 
@@ -148,15 +136,20 @@ def run_replicas(a):
     """Replica ensemble: 64 replicas, N_r ~ U{2000..50000}, Gamma_r in {10, 50,
     100, 175}, kappa_r in {0.5, 1, 2, 3} (RandomState(3) draws in that order),
     positions RandomState(1000 + r), velocities RandomState(2000 + r) at
-    T_r = 1/Gamma_r.  (mesh, alpha, rc) per replica from the autotuner for an
-    RMS force error <= 1e-5 of the mean force (FP32-mixed); the tuning is
-    outside the timed region.  Timed: `steps` NVE steps of all replicas with
-    n(q, t) (20 q per axis) recorded on device, one graph launch per
-    --graph-steps steps."""
+    T_r = 1/Gamma_r.  Each replica is first equilibrated from that start
+    (100 steps at dt/10, then --eq-steps steps with a velocity rescale every
+    10 steps and a per-ion speed clip, on a provisional setting: 32^3 or 64^3
+    mesh, rc = min(5, L/3), alpha = 3.2/rc), and its (mesh, alpha, rc) is
+    then taken from the autotuner on the equilibrated liquid, where an RMS
+    force error <= 1e-5 of the mean force is the harder bar (FP32-mixed).
+    Equilibration and tuning are outside the timed region.  Timed: `steps`
+    NVE steps of all replicas from their equilibrated states with n(q, t)
+    (20 q per axis) recorded on device, one graph launch per --graph-steps
+    steps."""
     import numpy as np
     import torch
 
-    from mdforce import autotune
+    from mdforce import autotune, mlp_train
     from mdforce.batch import ReplicaBatch
     from mdforce.engine import P3M
     from mdforce.initialize_pos_vel import initial_split
@@ -176,12 +169,19 @@ def run_replicas(a):
         L = (4.0 * math.pi * N / 3.0) ** (1.0 / 3.0)
         pos, vel = initial_split(1000 + r, 2000 + r, N, [L, L, L], 1.0 / gammas[r])
         p = torch.from_numpy(pos).to(dev)
+        v = torch.from_numpy(vel).to(dev)
+        rc_eq = min(5.0, L / 3.0 * (1.0 - 1e-9))
+        eq = P3M(N, 32 if N <= 10000 else 64, L, float(kappas[r]), 3.2 / rc_eq, rc_eq, precision="mixed",
+                 device=dev)
+        eq.build_green()
+        mlp_train.equilibrate(eq, p, v, 1.0 / gammas[r], a.eq_steps, a.dt, vcap=4.0, soft_steps=100)
+        eq.close()
         ch = autotune.tune(p, L, float(kappas[r]), budget=1e-5, precision="mixed", device=dev)
         errs.append(ch.error)
         systems.append(dict(n=N, L=L, kappa=float(kappas[r]), mesh=ch.mesh, alpha=ch.alpha, rc=ch.rc,
                             precision="mixed"))
         P.append(p)
-        V.append(torch.from_numpy(vel).to(dev))
+        V.append(v)
         print(f"[tune] replica {r}: N={N} Gamma={gammas[r]:g} kappa={kappas[r]:g} -> mesh {ch.mesh} "
               f"alpha {ch.alpha} rc {ch.rc} err {ch.error:.2e}", file=sys.stderr, flush=True)
     t_tune = time.perf_counter() - t_tune
@@ -228,14 +228,16 @@ def run_replicas(a):
         "data": "synthetic",
         "config": {
             "workload": f"{R} replicas, N {int(Ns.min())}-{int(Ns.max())} (sum {total_n}), "
-                        "Gamma {10,50,100,175}, kappa {0.5,1,2,3}, FP32-mixed, autotuned mesh/alpha/rc",
+                        "Gamma {10,50,100,175}, kappa {0.5,1,2,3}, FP32-mixed, equilibrated, autotuned "
+                        "mesh/alpha/rc",
             "graph_steps": K, "graph_mode": ("segmented" if b.graph_mode == 0 else "per-replica streams"),
             "streams": a.streams, "nq": nq, "dt": a.dt,
             "sum_N_rc3": float(sum(s["n"] * s["rc"] ** 3 for s in systems)),
             "tuned_error_max": max(errs), "tuned_error_mean": float(np.mean(errs)),
             "meshes": sorted(set(s["mesh"] for s in systems)),
             "rc_range": [min(s["rc"] for s in systems), max(s["rc"] for s in systems)],
-            "tune_s": t_tune, "wall_s": wall, "energy_drift_per_NT_max": max(drift), "nqt_finite": nq_ok,
+            "tuned_on": "equilibrated liquid", "eq_and_tune_s": t_tune, "wall_s": wall,
+            "energy_drift_per_NT_max": max(drift), "nqt_finite": nq_ok,
             "parallelism": "single GPU, replicas overlapped on worker streams inside one hipGraph",
         },
         "roofline": None,
@@ -357,7 +359,8 @@ def run_drift(a):
                     of the closest random pairs leaves no fast ions behind.
       From the equilibrated start each form runs --drift-steps NVE steps.
     Every 100 steps of every trajectory the force is compared with tight FP64
-    P3M (alpha 0.6, rc 9, 512^3) on the current positions.  Reported per start and form: max |E(t) - E(0)| /
+    P3M (autotune.reference_settings: 512^3, alpha 0.32, rc 15 at N = 1e6)
+    on the current positions.  Reported per start and form: max |E(t) - E(0)| /
     (N T), the slope of a linear fit of (E - E(0)) / (N T) per 1000 steps,
     the force error (RMS over mean |F|) every 100 steps, the largest speed at
     the end, and ms per step."""
@@ -374,7 +377,9 @@ def run_drift(a):
     L = (4.0 * math.pi * N / 3.0) ** (1.0 / 3.0)
     T = 1.0 / a.gamma
     pos0, vel0 = initial_split(1, 2, N, [L, L, L], T)
-    ref = P3M(N, 512, L, a.kappa, 0.6, 9.0, device=dev)
+    from mdforce.autotune import reference_settings
+    ref_set = reference_settings(L)
+    ref = P3M(N, ref_set[0], L, a.kappa, ref_set[1], ref_set[2], device=dev)
     ref.build_green()
     starts = {}
     if a.start in ("uniform", "both"):
@@ -498,10 +503,15 @@ def main():
     if a.autotune:
         from mdforce import autotune
         t_tune = time.perf_counter()
+        # the configured mesh first; twice as fine only if no (alpha, rc) meets the budget on it
         ch = autotune.tune(pos_liq if pos_liq is not None else torch.from_numpy(pos0).to(dev), L, a.kappa,
-                           budget=1e-5, meshes=(M,), fallback_meshes=(), precision="mixed", device=dev)
+                           budget=1e-5, meshes=(M,), fallback_meshes=(2 * M,), precision="mixed", device=dev)
         a.alpha, a.rc = ch.alpha, ch.rc
-        tuned = {"alpha": ch.alpha, "rc": ch.rc, "error": ch.error, "candidates": len(ch.tried),
+        if ch.mesh != M:
+            print(f"[bench] no (alpha, rc) on {M}^3 meets the budget on this state; tuned on {ch.mesh}^3",
+                  file=sys.stderr, flush=True)
+            M = a.mesh = ch.mesh
+        tuned = {"mesh": ch.mesh, "alpha": ch.alpha, "rc": ch.rc, "error": ch.error, "candidates": len(ch.tried),
                  "tuned_on": "equilibrated liquid" if pos_liq is not None else "uniform start",
                  "tune_s": time.perf_counter() - t_tune}
     if a.calib_copy:
@@ -512,8 +522,11 @@ def main():
         del src, dst
 
     refs = {}
+    ref_set = None
     if not a.no_err:
-        ref = P3M(N, 512, L, a.kappa, 0.6, 9.0, device=dev)
+        from mdforce.autotune import reference_settings
+        ref_set = reference_settings(L)
+        ref = P3M(N, ref_set[0], L, a.kappa, ref_set[1], ref_set[2], device=dev)
         ref.build_green(6, 3)
         refs["uniform_start"] = (torch.from_numpy(pos0).to(dev), ref.force(torch.from_numpy(pos0).to(dev))[0].clone())
         if pos_liq is not None:
@@ -534,11 +547,12 @@ def main():
             out[k] = {"rms_rel_rms": rms_d / rms_f, "rms_rel_meanF": rms_d / mean_f, "mean_F": mean_f}
         if out:
             out["max_rel_meanF"] = max(v["rms_rel_meanF"] for v in out.values())
-            out["reference"] = "FP64 P3M alpha 0.6, rc 9, 512^3"
+            out["reference"] = f"FP64 P3M {ref_set[0]}^3, alpha {ref_set[1]:.3f}, rc {ref_set[2]:.2f}"
         return out
 
     def run_form(precision):
         eng = P3M(N, M, L, a.kappa, a.alpha, a.rc, precision=precision, device=dev)
+        pp_cfg = eng.config()
         G_full, _ = eng.build_green(6, 3, want_full=True)
         pos = torch.from_numpy(pos0).to(dev)
         vel = torch.from_numpy(vel0).to(dev)
@@ -569,6 +583,7 @@ def main():
         drift = float(np.max(np.abs(en_h[:, 0] - en_h[0, 0])) / (N * T))
         eng.close()
         return {"gpu_ms": gpu_ms, "wall": wall, "stages": stages, "err": err, "drift": drift, "G_full": G_full,
+                "pp_config": pp_cfg,
                 "acc0": acc_init, "value": N * a.steps / (gpu_ms / 1e3)}
 
     forms = {"both": ("fp64", "mixed"), "chosen": ("mixed",), "plain": ("fp64",)}[a.form]
@@ -591,7 +606,7 @@ def main():
         roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = pmc_traffic(dom, main_form, N, M, a.rc, a.alpha)
-    pp_name = f"k_pp<{'float' if main_form == 'mixed' else 'double'}, {pp_waves(N, L, a.rc, main_form == 'mixed')}>"
+    pp_name = f"k_pp<{'float' if main_form == 'mixed' else 'double'}, {R['pp_config']['pp_waves']}>"
     roof["kernel"] = {"pp": pp_name, "fft": "fft passes", "gather": "k_gather", "assign": "k_assign",
                       "bin": "bin kernels", "drift": "k_kick_drift", "modes": "k_modes_grid"}.get(dom, dom)
     roof["avg_launch_ms"] = avg_s * 1e3
@@ -603,7 +618,8 @@ def main():
     for f, r in res.items():
         form_cfg["chosen" if f == "mixed" else "plain"] = {
             "particle_steps_per_s": r["value"], "ms_per_step": r["gpu_ms"] / a.steps, "force_error": r["err"],
-            "energy_drift_per_NT": r["drift"], "stage_ms": stage_ms(r["stages"]), "wall_s": r["wall"]}
+            "energy_drift_per_NT": r["drift"], "stage_ms": stage_ms(r["stages"]), "wall_s": r["wall"],
+            "pp_config": r["pp_config"]}
     if len(res) == 2:
         form_cfg["chosen_over_plain"] = res["mixed"]["value"] / res["fp64"]["value"]
     G_full = R["G_full"]

@@ -138,6 +138,11 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout);
  * slower), accumulated since create.  host pointer, 2 ints. */
 int mdf_p3m_stats(mdf_p3m_ctx *ctx, int *out2);
 
+/* Diagnostics (no reference counterpart): the PP launch configuration the
+ * context chose.  host pointer, 4 ints: waves per PP workgroup, LDS tile
+ * capacity in particles, pieces of the pair-term table, PP LDS bytes. */
+int mdf_p3m_config(mdf_p3m_ctx *ctx, int *out4);
+
 /* Replica ensembles (no single reference counterpart: the reference runs
  * each seeded replica of yukawa_MD_p3m_ord6.py:195-219 as its own process
  * and averages S(q,w) afterwards, s_of_q_w_avg.py).  A batch owns one

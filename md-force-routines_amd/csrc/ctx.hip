@@ -575,6 +575,18 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout) {
     return MDF_EINVAL;
 }
 
+int mdf_p3m_config(mdf_p3m_ctx *ctx, int *out4) {
+    if (!ctx || !out4) {
+        set_error("mdf_p3m_config: null argument");
+        return MDF_EINVAL;
+    }
+    out4[0] = ctx->pp_waves;
+    out4[1] = ctx->tile_max;
+    out4[2] = ctx->npiece;
+    out4[3] = (int)ctx->pp_lds;
+    return MDF_OK;
+}
+
 int mdf_p3m_stats(mdf_p3m_ctx *ctx, int *out2) {
     if (!ctx || !out2) {
         set_error("mdf_p3m_stats: null argument");

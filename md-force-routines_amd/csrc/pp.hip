@@ -549,12 +549,16 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
     return MDF_OK;
 }
 // A launch's LDS is the largest of its replicas' (each computed for that
-// replica's wave count).  Above 64 KB the group holds a 16-wave replica, whose
-// reservation also covers the 16-wave queue of any 4- or 8-wave replica beside
-// it (those have at most 64 KB + 6 KB with a 16-wave queue).
+// replica's wave count).  The per-wave scratch (the queues) is
+// kGroups * kQCap * 4 = 512 B.  A replica runs 16 waves when its 4-wave size
+// exceeds 64 KB, so its own size exceeds 70 KB, while 4- and 8-wave replicas
+// stay within 64 + 2 KB; above 68 KB the group therefore holds a 16-wave
+// replica, whose reservation also covers the 16-wave queues of any 4- or
+// 8-wave replica beside it (at most 64 + 6 KB).
+static_assert(kGroups * kQCap * 4 == 512, "per-wave PP scratch is 512 B");
 template <class R>
 static int launch_pp_seg(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
-    if (lds > 65536) {
+    if (lds > 68 * 1024) {
         MDF_TRY(allow_lds(k_pp_seg<R, 16>, lds));
         hipLaunchKernelGGL((k_pp_seg<R, 16>), dim3(total), dim3(16 * 64), lds, s, (const PPArgs<R> *)tab, S);
     } else if (sizeof(R) == 4) {

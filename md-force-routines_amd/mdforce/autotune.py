@@ -5,8 +5,8 @@ own configuration on the GPU) the autotuner chooses the mesh M, the Ewald
 parameter alpha and the PP cutoff rc with the lowest modelled step cost whose
 measured RMS force error stays within the budget.  The error of a candidate
 is measured, not estimated: the candidate's P3M force on the system's own
-positions is compared with a tight FP64 P3M force (alpha * rc >= 4.4, fine
-mesh) computed once per system.  The error measure is the one the force
+positions is compared with a tight FP64 P3M force (reference_settings:
+alpha rc = 4.8, alpha h <= 0.2) computed once per system.  The error measure is the one the force
 budget is stated in: sqrt(mean_i |F_i - F_i^ref|^2) / mean_i |F_i^ref|.
 
 Cost model (seconds per step), fitted to the N = 1e6, 128^3 stage times of
@@ -58,13 +58,25 @@ class Choice:
     tried: list = field(default_factory=list)
 
 
+def reference_settings(L):
+    """(mesh, alpha, rc) of the tight reference for a cubic box of side L:
+    rc = min(15, L/3), alpha = 4.8/rc (PP truncation ~exp(-23)), and the
+    smallest power-of-two mesh (32 .. 512) with alpha h <= 0.2.  Against
+    rc 20 / alpha 0.24 / 512^3 on equilibrated liquids this differs by 2e-7
+    of the mean force at N = 8e6 (kappa 0.5, h = 0.63) and 2e-13 at N = 1e6
+    (kappa 2); a spacing-only rule (alpha h = 0.32 at h = 0.63) was off by
+    1.2e-5 in the large box."""
+    rc = min(15.0, L / 3.0 * (1.0 - 1e-9))
+    alpha = 4.8 / rc
+    mesh = 32
+    while mesh < 512 and alpha * L / mesh > 0.2:
+        mesh *= 2
+    return mesh, alpha, rc
+
+
 def reference_force(P3M, pos, L, kappa, device=None):
     """Tight FP64 P3M force of a configuration (unit charges, af = 1)."""
-    rc = min(9.0, L / 3.0 * (1.0 - 1e-9))
-    alpha = 4.6 / rc
-    mesh = 64  # smallest power of two with spacing <= 0.65 (alpha h <= 0.33 at rc = 9), at most 512
-    while mesh < 512 and L / mesh > 0.65:
-        mesh *= 2
+    mesh, alpha, rc = reference_settings(L)
     e = P3M(pos.shape[0], mesh, L, kappa, alpha, rc, device=device)
     e.build_green()
     acc, _ = e.force(pos, None, af=1.0)

@@ -105,6 +105,13 @@ class P3M:
         check(self._lib.mdf_p3m_stats(self._ctx, out), "mdf_p3m_stats")
         return {"pp_multipass_tiles": out[0], "mesh_brick_outside": out[1]}
 
+    def config(self):
+        """PP launch configuration chosen at create: waves per workgroup, LDS
+        tile capacity (particles), pair-table pieces, LDS bytes."""
+        out = (ct.c_int * 4)()
+        check(self._lib.mdf_p3m_config(self._ctx, out), "mdf_p3m_config")
+        return {"pp_waves": out[0], "tile_max": out[1], "npiece": out[2], "pp_lds_bytes": out[3]}
+
     STAGES = ("bin", "pp", "assign", "fft", "gather", "drift", "modes", "finalize")
 
     def profile_start(self, events=65536):
