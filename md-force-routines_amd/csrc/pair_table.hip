@@ -4,8 +4,14 @@
 // (yukawa_pp.py:143-152) are fitted on [0, rc] by piecewise Chebyshev
 // series: npiece equal pieces of a given degree, coefficients from the
 // discrete cosine transform of long-double values at the Chebyshev nodes of
-// each piece.  The fit, with its coefficients rounded to the storage type
-// (FP64, or FP32 for the mixed-precision kernel), is then checked against
+// each piece.  FP64 tables keep the Chebyshev coefficients (Clenshaw in the
+// kernel); FP32 tables hold each piece in the power basis of the same local
+// variable t in [-1, 1] (Horner in the kernel: 14 FMAs for both terms
+// instead of two Clenshaw recurrences).  At degree 7 the power-basis
+// coefficients of these smooth pieces stay within a few times the function,
+// and FP32 Horner measured 5e-8 .. 8e-8 of max|g| where FP32 Clenshaw gave
+// 7e-8 .. 1.3e-7 (kappa 0.5 .. 3, alpha 0.25 .. 1.1, rc 3 .. 14).  The fit,
+// with its coefficients rounded to the storage type, is checked against
 // direct long-double evaluation at 64 points per piece; if the worst error
 // exceeds tol * max|g| on [0, rc], the piece count is doubled (up to 512).
 // FP64 tables use degree 13 and tol 4e-15; FP32 tables degree 7 and 1e-7.
@@ -41,6 +47,26 @@ static long double clenshaw_ld(const double2 *c, int deg, int comp, bool as_floa
     return t * b1 - b2 + coef(0);
 }
 
+static long double horner_ld(const double2 *c, int deg, int comp, long double t) {
+    long double a = (long double)(float)(comp ? c[deg].y : c[deg].x);
+    for (int k = deg - 1; k >= 0; --k) a = a * t + (long double)(float)(comp ? c[k].y : c[k].x);
+    return a;
+}
+
+// Power-basis coefficients m[j] of sum_k c[k] T_k(t), k, j <= deg
+static void cheb_to_power(const long double *c, int deg, long double *m) {
+    long double T[32][32] = {};
+    T[0][0] = 1.0L;
+    if (deg >= 1) T[1][1] = 1.0L;
+    for (int k = 2; k <= deg; ++k)
+        for (int j = 0; j <= k; ++j) T[k][j] = (j ? 2.0L * T[k - 1][j - 1] : 0.0L) - T[k - 2][j];
+    for (int j = 0; j <= deg; ++j) {
+        long double s = 0.0L;
+        for (int k = j; k <= deg; ++k) s += c[k] * T[k][j];
+        m[j] = s;
+    }
+}
+
 int build_pair_table(double kappa, double alpha, double rc, int deg, double tol, bool as_float,
                      std::vector<double2> &tab, int &npiece) {
     const int nc = deg + 1;
@@ -62,6 +88,7 @@ int build_pair_table(double kappa, double alpha, double rc, int deg, double tol,
                 const long double t = cosl(pi * (j + 0.5L) / nc);
                 pair_g(p * w + (t + 1.0L) * 0.5L * w, kappa, alpha, v1[j], v2[j]);
             }
+            long double c1[32], c2[32];
             for (int k = 0; k < nc; ++k) {
                 long double s1 = 0.0L, s2 = 0.0L;
                 for (int j = 0; j < nc; ++j) {
@@ -70,8 +97,19 @@ int build_pair_table(double kappa, double alpha, double rc, int deg, double tol,
                     s2 += v2[j] * cs;
                 }
                 const long double f = (k == 0 ? 1.0L : 2.0L) / nc;
-                tab[(size_t)p * nc + k] = make_double2((double)(f * s1), (double)(f * s2));
+                c1[k] = f * s1;
+                c2[k] = f * s2;
             }
+            if (as_float) {
+                long double m1[32], m2[32];
+                cheb_to_power(c1, deg, m1);
+                cheb_to_power(c2, deg, m2);
+                for (int k = 0; k < nc; ++k) {
+                    c1[k] = m1[k];
+                    c2[k] = m2[k];
+                }
+            }
+            for (int k = 0; k < nc; ++k) tab[(size_t)p * nc + k] = make_double2((double)c1[k], (double)c2[k]);
         }
         // check
         long double m1 = 0.0L, m2 = 0.0L, e1 = 0.0L, e2 = 0.0L;
@@ -81,7 +119,8 @@ int build_pair_table(double kappa, double alpha, double rc, int deg, double tol,
                 long double g1, g2;
                 pair_g(p * w + (t + 1.0L) * 0.5L * w, kappa, alpha, g1, g2);
                 const double2 *c = &tab[(size_t)p * nc];
-                const long double a1 = clenshaw_ld(c, deg, 0, as_float, t), a2 = clenshaw_ld(c, deg, 1, as_float, t);
+                const long double a1 = as_float ? horner_ld(c, deg, 0, t) : clenshaw_ld(c, deg, 0, false, t);
+                const long double a2 = as_float ? horner_ld(c, deg, 1, t) : clenshaw_ld(c, deg, 1, false, t);
                 m1 = fmaxl(m1, fabsl(g1));
                 m2 = fmaxl(m2, fabsl(g2));
                 e1 = fmaxl(e1, fabsl(a1 - g1));

@@ -113,6 +113,21 @@ __device__ __forceinline__ void cheb2(const T2 *__restrict__ c, T t, T &g1, T &g
     g2 = fma(t, b1, c[0].y - b2);
 }
 
+// Horner sums of both power series of one piece at t in [-1, 1] (FP32 tables).
+template <class T, class T2>
+__device__ __forceinline__ void horner2(const T2 *__restrict__ c, T t, T &g1, T &g2) {
+    constexpr int D = ChebDeg<T>::v;
+    T a = c[D].x, b = c[D].y;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+        const T2 ck = c[k];
+        a = fma(a, t, ck.x);
+        b = fma(b, t, ck.y);
+    }
+    g1 = a;
+    g2 = b;
+}
+
 template <class T>
 __device__ __forceinline__ T rsqrt_t(T x);
 template <>
@@ -137,7 +152,8 @@ __device__ __forceinline__ void pp_pair_cheb(const PPArgs<T> &P, const typename 
     p = p < P.npiece - 1 ? (p > 0 ? p : 0) : P.npiece - 1;
     const T t = (r - p * P.piece_w) * ((T)2 * P.inv_piece_w) - (T)1;
     T g1, g2;
-    cheb2<T>(tab + p * (D + 1), t, g1, g2);
+    if constexpr (sizeof(T) == 4) horner2<T>(tab + p * (D + 1), t, g1, g2);  // power basis (FP32 table)
+    else cheb2<T>(tab + p * (D + 1), t, g1, g2);
     u += g1 * ir;
     const T fr = g2 * ir * ir * ir;
     fx += fr * dx;
@@ -382,7 +398,6 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
             };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
-            int qmine = 0;         // queue fill of this lane's half
             for (int rr = 0; rr < 5; ++rr) {
                 int2 lr = rng[h][rr];
                 if (!have_i) lr = make_int2(0, 0);
@@ -405,19 +420,18 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         ddz = zi - tz[jc];
                     }
                     const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const bool hit =
-                        inrow && (r2 < rc2m) && (kRel ? (j != islot) : (r2 > (R)0));
+                    const bool hit = inrow & (r2 < rc2m) & (kRel ? (j != islot) : (r2 > (R)0));
                     const unsigned long long m = __ballot(hit);
                     const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
                     const int clo = __popc(mlo), chi = __popc(mhi);
-                    // rank among this half's hits: bits below the lane, minus the lower half for upper lanes
-                    int pos = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
-                    if (grp) pos -= clo;
-                    if (hit) q[qmine + pos] = j;
-                    qmine += grp ? chi : clo;
+                    // queue slot: the half's fill plus the hits on lower lanes of
+                    // the half (mbcnt counts all of the lower half for upper lanes)
+                    const int qb = grp ? qn1 - clo : qn0;
+                    if (hit) q[qb + __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u))] = j;
                     qn0 += clo;
                     qn1 += chi;
                     if (qn0 >= kGL || qn1 >= kGL) {
+                        const int qmine = grp ? qn1 : qn0;
                         const int cnt = min(qmine, kGL);
                         if (gl < cnt) {
                             const int jj = q[gl];
@@ -428,10 +442,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         if (gl < rest) q[gl] = moved;
                         qn0 = max(qn0 - kGL, 0);
                         qn1 = max(qn1 - kGL, 0);
-                        qmine = max(qmine - kGL, 0);
                     }
                 }
             }
+            const int qmine = grp ? qn1 : qn0;
             if (gl < qmine) {
                 const int jj = q[gl];
                 pair_at(jj);
