@@ -4,12 +4,15 @@
 // (yukawa_p3m_ord6.force_pot, yukawa_p3m_ord6.py:6-29) and its children; the
 // Python layer (mdforce.yukawa_p3m_ord6 and friends) keeps the reference's
 // positional signatures and calls these through ctypes.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <vector>
+
+#include <algorithm>
 
 #include "mdf_internal.h"
 
@@ -206,17 +209,39 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             return rr;
         }
     }
-    // waves per PP workgroup: 16 for tiles above 64 KB (one workgroup per CU);
-    // otherwise as many as the registers allow beside 4 LDS-bound workgroups
-    // per CU: 8 for the FP32 kernel (76 VGPR), 4 for FP64 (91 VGPR)
-    c->pp_waves = pp_lds_bytes(c->tile_max, c->npiece, mixed, 4) > 64 * 1024 ? 16 : (mixed ? 8 : 4);
-    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
-    // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB
-    const size_t lds_cap = 160 * 1024 - 4096;
-    if (c->pp_lds > lds_cap) {
-        c->tile_max -= (int)(((c->pp_lds - lds_cap) / (mixed ? 32 : 24) + 4) & ~(size_t)3);
-        c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
+    // waves per PP workgroup.  FP64 (95 VGPR): 16 for tiles above 64 KB (one
+    // workgroup per CU), else 4.  FP32 (78 VGPR, at most 6 waves a SIMD = 24 a
+    // CU): the count of 8, 12 or 16 that puts the most waves on a CU given the
+    // LDS of each workgroup (dynamic + ~1 KB static of 160 KB), the fewest
+    // waves on a tie; 16 when even one workgroup's tile overflows the LDS
+    // (multipass).  The 12-wave kernel packs its tile to 24 B a particle
+    // (32 B otherwise): at rc = 2w and unit density (2568-particle tiles) that
+    // is two workgroups of 12 waves, against one of 16.
+    if (mixed) {
+        int best = 16, best_waves = 0;
+        for (int nw : {8, 12, 16}) {
+            const size_t l = pp_lds_bytes(c->tile_max, c->npiece, true, nw) + 1024;
+            const int wg = (int)std::min<size_t>((160 * 1024) / l, (size_t)(24 / nw));
+            if (wg * nw > best_waves) {
+                best_waves = wg * nw;
+                best = nw;
+            }
+        }
+        c->pp_waves = best;
+    } else {
+        c->pp_waves = pp_lds_bytes(c->tile_max, c->npiece, false, 4) > 64 * 1024 ? 16 : 4;
     }
+    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
+    // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB, for
+    // this wave count and for that of segmented launches (16 above 64 KB)
+    const size_t lds_cap = 160 * 1024 - 4096;
+    auto lds_need = [&]() {
+        return std::max(pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves),
+                        pp_lds_bytes(c->tile_max, c->npiece, mixed, 16));
+    };
+    while (lds_need() > lds_cap && c->tile_max > 0)
+        c->tile_max -= (int)(((lds_need() - lds_cap) / 32 + 4) & ~(size_t)3);
+    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
     if (c->tile_max < 256) {
         delete c;
         set_error("PP pair table too large for LDS (kappa=%g alpha=%g rc=%g)", p->kappa, p->alpha, p->rc);

@@ -184,15 +184,21 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     using R2 = typename Vec2<R>::t;
     constexpr int D = ChebDeg<R>::v;
     constexpr bool kRel = sizeof(R) == 4;  // tile relative to the home block centre
+    // FP32 tile: hi and lo as two float4 (32 B a particle), or packed into a
+    // float4 (hi x, y, z, lo x) and a float2 (lo y, z) (24 B) for the 12-wave
+    // kernel, where the smaller tile is what lets two workgroups share a CU;
+    // at equal occupancy the packed pair read measured ~10% slower
+    constexpr bool kPack = kRel && NW == 12;
     const int T = P.tile_max;
     // FP64: three coordinate arrays (24 B a particle).  FP32: hi and lo as
     // two float4 per particle (32 B), one 16-byte LDS read each
     R *tx = (R *)ldsraw;
     R *ty = tx + T;
     R *tz = ty + T;
-    float4 *th4 = (float4 *)ldsraw;                        // mixed: (x, y, z, 0) hi
-    float4 *tl4 = th4 + T;                                 // mixed: lo parts
-    R2 *tab = kRel ? (R2 *)(tl4 + T) : (R2 *)(tz + T);     // npiece * (D + 1)
+    float4 *th4 = (float4 *)ldsraw;                        // mixed: hi (x, y, z), packed: + lo x
+    float4 *tl4 = th4 + T;                                 // mixed, unpacked: lo (x, y, z)
+    float2 *tl2 = (float2 *)(th4 + T);                     // mixed, packed: lo (y, z)
+    R2 *tab = kRel ? (kPack ? (R2 *)(tl2 + T) : (R2 *)(tl4 + T)) : (R2 *)(tz + T);  // npiece * (D + 1)
     int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // NW * kGroups * kQCap
     __shared__ int tcell[216];
@@ -305,9 +311,15 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                     if (kRel) {
                         const double cx = (p.x + sx) - ox, cy = (p.y + sy) - oy, cz = (p.z + sz) - oz;
                         const float hx = (float)cx, hy = (float)cy, hz = (float)cz;
-                        th4[dst + t] = make_float4(hx, hy, hz, 0.0f);
-                        tl4[dst + t] = make_float4((float)(cx - (double)hx), (float)(cy - (double)hy),
-                                                   (float)(cz - (double)hz), 0.0f);
+                        const float lx = (float)(cx - (double)hx), ly = (float)(cy - (double)hy),
+                                    lz = (float)(cz - (double)hz);
+                        if (kPack) {
+                            th4[dst + t] = make_float4(hx, hy, hz, lx);
+                            tl2[dst + t] = make_float2(ly, lz);
+                        } else {
+                            th4[dst + t] = make_float4(hx, hy, hz, 0.0f);
+                            tl4[dst + t] = make_float4(lx, ly, lz, 0.0f);
+                        }
                     } else {
                         tx[dst + t] = (R)(p.x + sx);
                         ty[dst + t] = (R)(p.y + sy);
@@ -358,13 +370,21 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             R xi, yi, zi, xil = 0, yil = 0, zil = 0;
             if (islot >= 0) {
                 if constexpr (kRel) {
-                    const float4 a = th4[islot], b = tl4[islot];
+                    const float4 a = th4[islot];
                     xi = a.x;
                     yi = a.y;
                     zi = a.z;
-                    xil = b.x;
-                    yil = b.y;
-                    zil = b.z;
+                    if constexpr (kPack) {
+                        const float2 b = tl2[islot];
+                        xil = a.w;
+                        yil = b.x;
+                        zil = b.y;
+                    } else {
+                        const float4 b = tl4[islot];
+                        xil = b.x;
+                        yil = b.y;
+                        zil = b.z;
+                    }
                 } else {
                     xi = tx[islot];
                     yi = ty[islot];
@@ -386,10 +406,18 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             auto pair_at = [&](int jj) {
                 R dx, dy, dz;
                 if constexpr (kRel) {
-                    const float4 a = th4[jj], b = tl4[jj];
-                    dx = (xi - a.x) + (xil - b.x);
-                    dy = (yi - a.y) + (yil - b.y);
-                    dz = (zi - a.z) + (zil - b.z);
+                    const float4 a = th4[jj];
+                    if constexpr (kPack) {
+                        const float2 b = tl2[jj];
+                        dx = (xi - a.x) + (xil - a.w);
+                        dy = (yi - a.y) + (yil - b.x);
+                        dz = (zi - a.z) + (zil - b.y);
+                    } else {
+                        const float4 b = tl4[jj];
+                        dx = (xi - a.x) + (xil - b.x);
+                        dy = (yi - a.y) + (yil - b.y);
+                        dz = (zi - a.z) + (zil - b.z);
+                    }
                 } else {
                     dx = xi - tx[jj];
                     dy = yi - ty[jj];
@@ -500,10 +528,11 @@ __global__ __launch_bounds__(NW * 64) void k_pp_seg(const PPArgs<R> *__restrict_
 }
 
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
-    // tile: (x, y, z) in FP64 (24 B a particle), or hi and lo float4 in FP32 (32 B)
+    // tile: (x, y, z) in FP64 (24 B a particle); FP32 hi and lo float4 (32 B),
+    // packed to 24 B in the 12-wave kernel
     const size_t es = mixed ? 4 : 8;
     const size_t ncoef = (size_t)npiece * ((mixed ? kChebDegF : kChebDeg) + 1);
-    return (mixed ? 32 : 24) * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + nw * kGroups * kQCap);
+    return ((mixed && nw != 12) ? 32 : 24) * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + nw * kGroups * kQCap);
 }
 
 template <class R>
@@ -532,6 +561,9 @@ static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_
     if (c->pp_waves == 16) {
         MDF_TRY(allow_lds(k_pp<R, 16>, c->pp_lds));
         hipLaunchKernelGGL((k_pp<R, 16>), dim3(c->nblock), dim3(16 * 64), c->pp_lds, s, P);
+    } else if (c->pp_waves == 12) {
+        MDF_TRY(allow_lds(k_pp<R, 12>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 12>), dim3(c->nblock), dim3(12 * 64), c->pp_lds, s, P);
     } else if (c->pp_waves == 8) {
         MDF_TRY(allow_lds(k_pp<R, 8>, c->pp_lds));
         hipLaunchKernelGGL((k_pp<R, 8>), dim3(c->nblock), dim3(8 * 64), c->pp_lds, s, P);
@@ -559,11 +591,15 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
     *(PPArgs<R> *)args = pp_args<R>(c, sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->chebf
                                                     : (const typename Vec2<R>::t *)c->cheb);
     dims = make_int4(c->nblock, 1, 1, 0);
-    lds = c->pp_lds;
+    // segmented launches run 16 waves for tiles above 64 KB, else 8 (FP32) or
+    // 4 (FP64), whatever the replica's own launch uses; the LDS is sized for that
+    const bool mixed = sizeof(R) == 4;
+    const int nw = pp_lds_bytes(c->tile_max, c->npiece, mixed, 4) > 64 * 1024 ? 16 : (mixed ? 8 : 4);
+    lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, nw);
     return MDF_OK;
 }
-// A launch's LDS is the largest of its replicas' (each computed for that
-// replica's wave count).  The per-wave scratch (the queues) is
+// A launch's LDS is the largest of its replicas' (each computed for the
+// replica's segmented wave count, build_pp).  The per-wave scratch (the queues) is
 // kGroups * kQCap * 4 = 512 B.  A replica runs 16 waves when its 4-wave size
 // exceeds 64 KB, so its own size exceeds 70 KB, while 4- and 8-wave replicas
 // stay within 64 + 2 KB; above 68 KB the group therefore holds a 16-wave
