@@ -136,7 +136,9 @@ __device__ __forceinline__ double rsqrt_t<double>(double x) {
 }
 template <>
 __device__ __forceinline__ float rsqrt_t<float>(float x) {
-    return rsqrtf(x);
+    // v_rsq_f32 (1 ulp) without rsqrtf's denormal rescaling: a pair's r^2 is a
+    // normal number (positions are O(L), rc^2 >> FLT_MIN)
+    return __builtin_amdgcn_rsqf(x);
 }
 
 template <class T>
@@ -448,8 +450,11 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         ddz = zi - tz[jc];
                     }
                     const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const bool hit = inrow & (r2 < rc2m) & (kRel ? (j != islot) : (r2 > (R)0));
-                    const unsigned long long m = __ballot(hit);
+                    const bool c1 = r2 < rc2m, c2 = kRel ? (j != islot) : (r2 > (R)0);
+                    const bool hit = inrow & c1 & c2;
+                    // the mask as the AND of the compares' own masks: no VALU to materialise it
+                    const unsigned long long m = __builtin_amdgcn_ballot_w64(inrow) & __builtin_amdgcn_ballot_w64(c1) &
+                                                 __builtin_amdgcn_ballot_w64(c2);
                     const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
                     const int clo = __popc(mlo), chi = __popc(mhi);
                     // queue slot: the half's fill plus the hits on lower lanes of
