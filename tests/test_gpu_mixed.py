@@ -127,6 +127,40 @@ def test_mixed_n1e5_error_vs_fp64(P3M):
     assert abs(u32[0] - u64[0]) <= ENERGY_TOL * abs(u64[0])
 
 
+@pytest.mark.parametrize("rc, waves", [(5.5, 8), (7.0, 12), (8.5, 16)])
+def test_mixed_pp_wave_counts_vs_fp64(P3M, rc, waves):
+    """Each FP32 PP kernel variant against the FP64 GPU path at unit density:
+    8 waves (rc 5.5), 12 waves with the packed 24-byte tile (rc 7) and 16
+    waves with tiles over the LDS capacity, i.e. several passes (rc 8.5).
+    N = 6e4 (L = 38.5): mean tiles of 1217, 2222 and 4723 particles, close to
+    those of N = 1e6 at these rc."""
+    N = 60000
+    L = O.box_length(N)
+    kappa, alpha, M = 2.0, 2.8 / rc, 64
+    rng = np.random.default_rng(11)
+    pos = dev(rng.random((N, 3)) * L)
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        e = P3M(N, M, L, kappa, alpha, rc, precision=prec)
+        if prec == "mixed":
+            cfg = e.config()
+            assert cfg["pp_waves"] == waves, cfg
+        e.build_green()
+        acc, en = e.force(pos, None)
+        torch.cuda.synchronize()
+        outs[prec] = (acc.cpu().numpy(), en.cpu().numpy(), e.stats())
+        e.close()
+    a64, u64, _ = outs["fp64"]
+    a32, u32, st = outs["mixed"]
+    err = rel_rms(a32, a64)
+    print(f"rc {rc}: {waves} waves, multipass tiles {st['pp_multipass_tiles']}: rel rms {err:.3e}, "
+          f"dU {abs(u32[0] - u64[0]) / abs(u64[0]):.3e}")
+    if waves == 16:
+        assert st["pp_multipass_tiles"] > 0
+    assert err <= FORCE_TOL
+    assert abs(u32[0] - u64[0]) <= ENERGY_TOL * abs(u64[0])
+
+
 def test_mixed_md_run_tracks_fp64(P3M, cfg1):
     """20 NVE steps from the same state: mixed trajectory stays close to FP64.
 
