@@ -180,15 +180,16 @@ __device__ __forceinline__ void modes_grid_body(const ModesArgs &A, const Blk &b
     double ar[kModeChunk], ai[kModeChunk];
 #pragma unroll
     for (int k = 0; k < kModeChunk; ++k) ar[k] = ai[k] = 0.0;
+    const double dqpi = dq * (1.0 / M_PI);  // angles in units of pi: sincospi reduces exactly
     for (int j = b.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        const double th = dq * pos[3 * j + a];
+        const double th = dqpi * pos[3 * j + a];
         double s1, c1, zs, zc;
-        sincos(-th, &s1, &c1);
+        sincospi(-th, &s1, &c1);
         if (m0 == 0) {
             zs = s1;
             zc = c1;
         } else {
-            sincos(-(double)(m0 + 1) * th, &zs, &zc);
+            sincospi(-(double)(m0 + 1) * th, &zs, &zc);
         }
         ar[0] += zc;
         ai[0] += zs;
@@ -421,15 +422,23 @@ int density_modes(mdf_p3m_ctx *c, const double *pos, int n, const double *q, int
     return MDF_OK;
 }
 
+// Blocks per axis of k_modes_grid: about 16 particles per thread (at most 256
+// blocks), so that the 2 x kModeChunk wave sums at the end of each wave cost
+// little beside its particles' work.  With one particle in four threads
+// (1024 blocks at N = 1e6) the sums took as long as the sincos and recurrences.
+static int modes_blocks(int n) {
+    int nb = (n + 4095) / 4096;
+    if (nb > 256) nb = 256;
+    return nb < 1 ? 1 : nb;
+}
+
 int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int nq, double *out, hipStream_t s) {
     if (nq <= 0) return MDF_OK;
     if (nq > 64) {
         set_error("density modes on a q grid: nq = %d > 64", nq);
         return MDF_EINVAL;
     }
-    int nb = (n + 255) / 256;
-    if (nb > 1024) nb = 1024;
-    if (nb < 1) nb = 1;
+    const int nb = modes_blocks(n);
     const int nw = nb * 4;
     const int nchunk = (nq + kModeChunk - 1) / kModeChunk;
     const ModesArgs A{pos, n, dq, nq, c->part_red};
@@ -509,11 +518,6 @@ static int launch_final(const void *tab, const SegGrid &S, int total, size_t, in
     hipLaunchKernelGGL(k_finalize_seg, dim3(total), dim3(1024), 0, s, (const FinalArgs *)tab, S, k);
     MDF_LAUNCHED();
     return MDF_OK;
-}
-static int modes_blocks(int n) {
-    int nb = (n + 255) / 256;
-    if (nb > 1024) nb = 1024;
-    return nb < 1 ? 1 : nb;
 }
 static int build_modes(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
     if (io.nq > 64) {
