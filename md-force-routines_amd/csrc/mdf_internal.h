@@ -68,7 +68,7 @@ struct Brick {
 constexpr int kPMThreads = 512;
 constexpr int kGatherThreads = 1024;
 constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces (FP64)
-constexpr int kChebDegF = 7;        // degree of the FP32 pieces (mixed precision)
+constexpr int kChebDegF = 4;        // degree of the FP32 pieces (mixed precision)
 constexpr int kMaxPartials = 1 << 20;
 
 // ctx->stats (device ints): [0] PP tiles needing more than one LDS pass,

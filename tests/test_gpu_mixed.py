@@ -130,21 +130,28 @@ def test_mixed_n1e5_error_vs_fp64(P3M):
 @pytest.mark.parametrize("rc, waves", [(5.5, 8), (7.0, 12), (8.5, 16)])
 def test_mixed_pp_wave_counts_vs_fp64(P3M, rc, waves):
     """Each FP32 PP kernel variant against the FP64 GPU path at unit density:
-    8 waves (rc 5.5), 12 waves with the packed 24-byte tile (rc 7) and 16
-    waves with tiles over the LDS capacity, i.e. several passes (rc 8.5).
-    N = 6e4 (L = 38.5): mean tiles of 1217, 2222 and 4723 particles, close to
-    those of N = 1e6 at these rc."""
-    N = 60000
-    L = O.box_length(N)
-    kappa, alpha, M = 2.0, 2.8 / rc, 64
+    8 waves, 12 waves with the packed 24-byte tile, and 16 waves with tiles
+    over the LDS capacity (several passes).  The wave count follows from the
+    tile size, which depends on the cell geometry (floor(L/rc)); N is taken as
+    the first of a few sizes whose context picks the wanted count."""
+    kappa = 2.0
+    alpha = 2.8 / rc
+    chosen = None
+    for N in (60000, 50000, 80000, 40000, 100000, 30000, 120000, 150000):
+        L = O.box_length(N)
+        e = P3M(N, 64, L, kappa, alpha, rc, precision="mixed")
+        if e.config()["pp_waves"] == waves:
+            chosen = (N, L, e)
+            break
+        e.close()
+    if chosen is None:
+        pytest.fail(f"no size picks {waves} waves at rc {rc}")
+    N, L, e_mixed = chosen
     rng = np.random.default_rng(11)
     pos = dev(rng.random((N, 3)) * L)
     outs = {}
     for prec in ("fp64", "mixed"):
-        e = P3M(N, M, L, kappa, alpha, rc, precision=prec)
-        if prec == "mixed":
-            cfg = e.config()
-            assert cfg["pp_waves"] == waves, cfg
+        e = e_mixed if prec == "mixed" else P3M(N, 64, L, kappa, alpha, rc, precision=prec)
         e.build_green()
         acc, en = e.force(pos, None)
         torch.cuda.synchronize()
@@ -153,7 +160,7 @@ def test_mixed_pp_wave_counts_vs_fp64(P3M, rc, waves):
     a64, u64, _ = outs["fp64"]
     a32, u32, st = outs["mixed"]
     err = rel_rms(a32, a64)
-    print(f"rc {rc}: {waves} waves, multipass tiles {st['pp_multipass_tiles']}: rel rms {err:.3e}, "
+    print(f"rc {rc}, N {N}: {waves} waves, multipass tiles {st['pp_multipass_tiles']}: rel rms {err:.3e}, "
           f"dU {abs(u32[0] - u64[0]) / abs(u64[0]):.3e}")
     if waves == 16:
         assert st["pp_multipass_tiles"] > 0
