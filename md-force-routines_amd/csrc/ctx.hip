@@ -166,6 +166,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         g.L[d] = p->box[d];
         g.M[d] = p->mesh[d];
         g.h[d] = p->box[d] / p->mesh[d];
+        g.ih[d] = 1.0 / g.h[d];
         const int lxd = (int)floor(p->box[d] / p->rc);  // reference cells, yukawa_pp.py:45-47
         g.nsub[d] = 2 * lxd;
         g.nblk[d] = lxd;

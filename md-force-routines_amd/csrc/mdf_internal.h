@@ -45,6 +45,7 @@ void set_error(const char *fmt, ...);
 struct Geom {
     double L[3];      // box
     double h[3];      // mesh spacing L/M
+    double ih[3];     // 1/h
     double w[3];      // PP sub-cell width L/nsub
     int M[3];         // mesh points
     int Mxh;          // M[0]/2 + 1 (half-spectrum length along x)

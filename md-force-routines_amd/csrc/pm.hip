@@ -36,12 +36,19 @@ __device__ __forceinline__ void weights6(T x, T w[6]) {
     w[5] = ((T)1 + x * ((T)10 + x * ((T)40 + x * ((T)80 + x * ((T)80 + x * (T)32))))) * (T)(1.0 / 3840.0);
 }
 
-// Base index floor(x/h) and the weights of the offset x/h - (ix + 1/2); the
-// index and offset are formed in FP64 from the FP64 position.
+// Base index floor(x/h) and the weights of the offset x/h - (ix + 1/2)
+// (yukawa_pm_ord6.py:17-27 takes x/h once and subtracts the same way); the
+// index and offset are formed in FP64 from the FP64 position, with x/h as
+// x * (1/h): one multiply instead of an FP64 division (each of a particle's
+// six lanes forms all three axes).  The two differ by an ulp of x/h; at a
+// node boundary that can move ix by one with the offset at +-1/2, where the
+// order-6 weights of the outermost node are zero, so the deposit agrees to
+// rounding either way.
 template <class T>
-__device__ __forceinline__ int base_and_weights(double x, double h, T w[6]) {
-    const int i0 = (int)floor(x / h);
-    const double fr = (x - (i0 + 0.5) * h) / h;
+__device__ __forceinline__ int base_and_weights(double x, double ih, T w[6]) {
+    const double s = x * ih;
+    const int i0 = (int)floor(s);
+    const double fr = s - (i0 + 0.5);
     weights6<T>((T)fr, w);
     return i0;
 }
@@ -145,9 +152,9 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
         if (sub >= 6) continue;
         const double4 p = P.spos[pm_particle(run_start, run_off, nrun, k)];
         T wx[6], wy[6], wz[6];
-        const int ix = base_and_weights<T>(p.x, g.h[0], wx);
-        const int iy = base_and_weights<T>(p.y, g.h[1], wy);
-        const int iz = base_and_weights<T>(p.z, g.h[2], wz);
+        const int ix = base_and_weights<T>(p.x, g.ih[0], wx);
+        const int iy = base_and_weights<T>(p.y, g.ih[1], wy);
+        const int iz = base_and_weights<T>(p.z, g.ih[2], wz);
         const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
         const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
         T wza = wz[0];
@@ -291,9 +298,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
             p = P.spos[s];
             if (sub < 6) {
                 T wx[6], wy[6], wz[6];
-                const int ix = base_and_weights<T>(p.x, g.h[0], wx);
-                const int iy = base_and_weights<T>(p.y, g.h[1], wy);
-                const int iz = base_and_weights<T>(p.z, g.h[2], wz);
+                const int ix = base_and_weights<T>(p.x, g.ih[0], wx);
+                const int iy = base_and_weights<T>(p.y, g.ih[1], wy);
+                const int iz = base_and_weights<T>(p.z, g.ih[2], wz);
                 T wza = wz[0];
 #pragma unroll
                 for (int a = 1; a < 6; ++a)
