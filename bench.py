@@ -253,10 +253,11 @@ def run_mlp(a):
     tight FP64 P3M (rc 8, 128^3) and the reduced setting (rc 3.5, 64^3,
     alpha 0.6; mesh spacing 1.26, that of N = 1e6 on 128^3).  Evaluation at
     N = 1e6, Gamma 100, kappa 2, equilibrated: RMS force error against a tight
-    FP64 P3M (rc 9, alpha 0.5, 256^3) of the reduced setting with and without
-    the correction, the plain P3M judge's estimate and the resulting form,
-    and the NVE throughput of the corrected reduced step and of the plain
-    FP64 step."""
+    FP64 P3M (autotune.reference_settings) of the reduced setting with and
+    without the correction, the estimate of the plain P3M judge (128^3 at the
+    bench's alpha and rc, which meet 1e-5 in the liquid) and the resulting
+    form, and the NVE throughput of the corrected reduced step and of the
+    plain FP64 step."""
     import numpy as np
     import torch
 
@@ -287,8 +288,9 @@ def run_mlp(a):
     ev = T.make_snapshots(N, [(gamma, kappa)], 1, 20, 300, red=lambda k: (128, a_red, rc_red),
                           tight=lambda k: (256, 0.5, 9.0), prop=lambda k: (128, 0.48, 5.5), seed0=5000, log=log)[0]
     pos = ev["pos"]
+    from mdforce.autotune import reference_settings
     Sr = T.Setting(N, 128, a_red, rc_red, kappa)
-    St = T.Setting(N, 256, 0.5, 9.0, kappa)
+    St = T.Setting(N, *reference_settings(T.box_length(N)), kappa)
     Fref = T.forces(St.eng, pos)
     St.close()
     meanF = float(torch.sqrt((Fref * Fref).sum(1)).mean())
@@ -297,7 +299,7 @@ def run_mlp(a):
     Fc = Fr + net.correction(Sr.eng, pos)
     e_red = float(torch.sqrt(((Fr - Fref) ** 2).sum(1).mean())) / meanF
     e_cor = float(torch.sqrt(((Fc - Fref) ** 2).sum(1).mean())) / meanF
-    Sp = T.Setting(N, 128, 0.48, 5.5, kappa)
+    Sp = T.Setting(N, 128, a.alpha, a.rc, kappa)
     form, est = T.choose_form(net, Sr.eng, Sp.eng, pos, kappa, a_red, 1e-5)
 
     def timed(eng, steps):

@@ -16,7 +16,7 @@
 // * the full-spectrum energy sum equals the half-spectrum sum with weight 2
 //   on 0 < kx < Mx/2 and 1 on kx = 0, Mx/2.
 //
-// Passes (complex FP64, or FP32 in the mixed mode; radix-2 Stockham in LDS,
+// Passes (complex FP64, or FP32 in the mixed mode; radix-4/2 Stockham in LDS,
 // twiddles from a table; the PM energy is always summed in FP64):
 //   1. x, real-to-complex: two real rows packed as one complex row.
 //   2. y, forward, in place on the half spectrum.
@@ -52,15 +52,78 @@ __device__ __forceinline__ C cmul(C a, C b) {
     return r;
 }
 
-// Radix-2 Stockham transform of nl lines of length m = 2^lm stored contiguously
-// in LDS (line stride m).  a holds the input, b is scratch; returns the buffer
-// holding the result.  tw[t] = exp(-2 pi i t / m), t < m/2.  DIR = -1 forward
-// (exp(-i...)), +1 inverse (unnormalised).  Ends with a barrier.
+// Stockham transform of nl lines of length m = 2^lm stored contiguously in
+// LDS (line stride m): radix-4 stages, then one radix-2 stage when lm is odd
+// (each stage reads and writes every element once, behind a barrier, so
+// radix 4 halves the LDS passes and barriers of a radix-2 transform).  a
+// holds the input, b is scratch; returns the buffer holding the result.
+// tw[t] = exp(-2 pi i t / m), t < m/2.  DIR = -1 forward (exp(-i...)), +1
+// inverse (unnormalised).  A stage of span Ns (the product of the radices
+// before it) takes element j of each of the R quarter (half) lines, turns
+// them by exp(DIR 2 pi i r k / (R Ns)), k = j mod Ns, applies the R-point
+// DFT and writes them to (j - k) R + k + r Ns.  Ends with a barrier.
 template <int DIR, class C>
 __device__ C *stockham(C *a, C *b, int nl, int m, int lm, const C *__restrict__ tw) {
-    const int half = m >> 1;
-    const int total = nl * half;
-    for (int s = 0; s < lm; ++s) {
+    int s = 0;  // log2 of the span Ns
+    if (lm >= 2) {
+        const int q = m >> 2;
+        const int total = nl * q;
+        for (; s + 2 <= lm; s += 2) {
+            const int Ns = 1 << s;
+            const int st = lm - 2 - s;  // m / (4 Ns) = 2^st
+            for (int t = threadIdx.x; t < total; t += blockDim.x) {
+                const int line = t >> (lm - 2);
+                const int j = t & (q - 1);
+                const C *src = a + line * m;
+                C *dst = b + line * m;
+                const int k = j & (Ns - 1);
+                C a0 = src[j], a1 = src[j + q], a2 = src[j + 2 * q], a3 = src[j + 3 * q];
+                if (s > 0) {
+                    C w1 = tw[k << st], w2 = tw[(2 * k) << st];  // indices < m/2
+                    if (DIR > 0) {
+                        w1.y = -w1.y;
+                        w2.y = -w2.y;
+                    }
+                    const C w3 = cmul(w1, w2);
+                    a1 = cmul(a1, w1);
+                    a2 = cmul(a2, w2);
+                    a3 = cmul(a3, w3);
+                }
+                C b0, b1, b2, b3;
+                b0.x = a0.x + a2.x;
+                b0.y = a0.y + a2.y;
+                b1.x = a0.x - a2.x;
+                b1.y = a0.y - a2.y;
+                b2.x = a1.x + a3.x;
+                b2.y = a1.y + a3.y;
+                b3.x = a1.x - a3.x;
+                b3.y = a1.y - a3.y;
+                const int o = ((j - k) << 2) + k;
+                C y;
+                y.x = b0.x + b2.x;
+                y.y = b0.y + b2.y;
+                dst[o] = y;
+                y.x = b0.x - b2.x;
+                y.y = b0.y - b2.y;
+                dst[o + 2 * Ns] = y;
+                // forward: y1 = b1 - i b3, y3 = b1 + i b3; inverse the other way round
+                C yp, ym;
+                yp.x = b1.x - b3.y;  // b1 + i b3
+                yp.y = b1.y + b3.x;
+                ym.x = b1.x + b3.y;  // b1 - i b3
+                ym.y = b1.y - b3.x;
+                dst[o + Ns] = DIR < 0 ? ym : yp;
+                dst[o + 3 * Ns] = DIR < 0 ? yp : ym;
+            }
+            __syncthreads();
+            C *t2 = a;
+            a = b;
+            b = t2;
+        }
+    }
+    if (s < lm) {  // last stage, radix 2 (span m/2)
+        const int half = m >> 1;
+        const int total = nl * half;
         const int Ns = 1 << s;
         for (int t = threadIdx.x; t < total; t += blockDim.x) {
             const int line = t >> (lm - 1);
@@ -74,13 +137,13 @@ __device__ C *stockham(C *a, C *b, int nl, int m, int lm, const C *__restrict__ 
             if (DIR > 0) w.y = -w.y;
             v = cmul(v, w);
             const int o = ((j - k) << 1) + k;
-            C p, q;
+            C p, r;
             p.x = u.x + v.x;
             p.y = u.y + v.y;
-            q.x = u.x - v.x;
-            q.y = u.y - v.y;
+            r.x = u.x - v.x;
+            r.y = u.y - v.y;
             dst[o] = p;
-            dst[o + Ns] = q;
+            dst[o + Ns] = r;
         }
         __syncthreads();
         C *t2 = a;
