@@ -9,11 +9,12 @@ positions is compared with a tight FP64 P3M force (reference_settings:
 alpha rc = 4.8, alpha h <= 0.2) computed once per system.  The error measure is the one the force
 budget is stated in: sqrt(mean_i |F_i - F_i^ref|^2) / mean_i |F_i^ref|.
 
-Cost model (seconds per step), fitted to the N = 1e6, 128^3 stage times of
-the FP32-mixed engine on MI355X:
+Cost model (seconds per step), fitted to the N = 1e6, 128^3, rc 7 stage
+times of the FP32-mixed engine on MI355X (PP 1.60 ms, assignment + gather
+0.37 ms, FFT solve 0.106 ms):
     t = a_pp * N * rc^3 + a_pm * N + a_fft * M^3 * log2(M^3) / 21 + t_0
-with a_pp = 1.1e-11 (PP at unit density: rc^3 in-cutoff pairs per
-particle), a_pm = 3.8e-10 (assignment + gather), a_fft = 6.1e-11 (FFT
+with a_pp = 4.7e-12 (PP at unit density: rc^3 in-cutoff pairs per
+particle), a_pm = 3.7e-10 (assignment + gather), a_fft = 5.0e-11 (FFT
 solve per mesh node at M = 128, scaled by log2(M^3)/21), t_0 = 2e-5 (launch
 overheads per step of a replica).
 
@@ -28,9 +29,9 @@ import torch
 
 from ._lib import MdforceError
 
-A_PP = 1.1e-11
-A_PM = 3.8e-10
-A_FFT = 6.1e-11
+A_PP = 4.7e-12
+A_PM = 3.7e-10
+A_FFT = 5.0e-11
 T_0 = 2e-5
 
 
