@@ -178,6 +178,24 @@ __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, doubl
 
 __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// Sum of v over each 32-lane half of the wave, returned in every lane of the
+// half.  DPP moves (VALU, no LDS): xor 1 and xor 2 by quad permutes, then the
+// half-row and row mirrors (sums of 8, then 16 lanes), then row_bcast:15
+// adds row 0 into row 1 and row 2 into row 3; one shuffle takes each half's
+// total from its last lane.  All lanes must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+__device__ __forceinline__ float half_wave_sum(float v) {
+    v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_add<0x140, 0xF>(v);  // row_mirror
+    v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    return __shfl(v, kGL - 1, kGL);
+}
+
 // NW waves per workgroup share the tile: 4 for tiles up to 64 KB, 16 for
 // larger ones (large rc), where one workgroup per CU would otherwise leave a
 // single wave per SIMD.
@@ -483,13 +501,25 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 const int jj = q[gl];
                 pair_at(jj);
             }
-            double sfx = fx, sfy = fy, sfz = fz, su = u;
+            double sfx, sfy, sfz, su;
+            if constexpr (sizeof(R) == 4) {
+                // FP32 sums of the lanes' FP32 partials (rounding ~1e-7 of the force)
+                sfx = half_wave_sum(fx);
+                sfy = half_wave_sum(fy);
+                sfz = half_wave_sum(fz);
+                su = half_wave_sum(u);
+            } else {
+                sfx = fx;
+                sfy = fy;
+                sfz = fz;
+                su = u;
 #pragma unroll
-            for (int o = kGL / 2; o > 0; o >>= 1) {
-                sfx += __shfl_xor(sfx, o, 64);
-                sfy += __shfl_xor(sfy, o, 64);
-                sfz += __shfl_xor(sfz, o, 64);
-                su += __shfl_xor(su, o, 64);
+                for (int o = kGL / 2; o > 0; o >>= 1) {
+                    sfx += __shfl_xor(sfx, o, 64);
+                    sfy += __shfl_xor(sfy, o, 64);
+                    sfz += __shfl_xor(sfz, o, 64);
+                    su += __shfl_xor(su, o, 64);
+                }
             }
             if (gl == 0 && have_i) {
                 double *a = P.acc + 3 * (size_t)gi;
