@@ -9,8 +9,9 @@
 // Work is organised per PM block of CB^3 reference cells (CB chosen per
 // context so the block's mesh brick fits LDS; the block's particles are
 // contiguous runs of the sorted array).  Each particle gets a group of
-// eight lanes and six of them take one z-plane (36 nodes) of its 6x6x6
-// stencil:
+// lanes: eight for the scatter, six of them taking one z-plane (36 nodes) of
+// its 6x6x6 stencil each; four for the gather, each taking a quarter (three
+// z-planes by three y-rows of six x nodes):
 // * scatter: the block's particles deposit into a private LDS brick of the
 //   mesh with LDS atomics, then the brick is added to the global mesh with
 //   FP64 atomics (one per non-empty brick node instead of 216 per particle);
@@ -25,31 +26,42 @@
 
 namespace mdf {
 
-// Weights of yukawa_pm_ord6.py:29-48 in Horner form, in arithmetic T.
+// Weights of yukawa_pm_ord6.py:29-48 in Horner form, in arithmetic T.  The
+// six polynomials pair up as w[5-k](x) = w[k](-x) (the coefficients of each
+// pair differ in the sign of the odd powers), so w0, w1, w2 give all six.
+template <class T>
+__device__ __forceinline__ T w6_0(T x) {
+    return ((T)1 + x * ((T)-10 + x * ((T)40 + x * ((T)-80 + x * ((T)80 + x * (T)-32))))) * (T)(1.0 / 3840.0);
+}
+template <class T>
+__device__ __forceinline__ T w6_1(T x) {
+    return ((T)237 + x * ((T)-750 + x * ((T)840 + x * ((T)-240 + x * ((T)-240 + x * (T)160))))) * (T)(1.0 / 3840.0);
+}
+template <class T>
+__device__ __forceinline__ T w6_2(T x) {
+    return ((T)841 + x * ((T)-770 + x * ((T)-440 + x * ((T)560 + x * ((T)80 + x * (T)-160))))) * (T)(1.0 / 1920.0);
+}
 template <class T>
 __device__ __forceinline__ void weights6(T x, T w[6]) {
-    w[0] = ((T)1 + x * ((T)-10 + x * ((T)40 + x * ((T)-80 + x * ((T)80 + x * (T)-32))))) * (T)(1.0 / 3840.0);
-    w[1] = ((T)237 + x * ((T)-750 + x * ((T)840 + x * ((T)-240 + x * ((T)-240 + x * (T)160))))) * (T)(1.0 / 3840.0);
-    w[2] = ((T)841 + x * ((T)-770 + x * ((T)-440 + x * ((T)560 + x * ((T)80 + x * (T)-160))))) * (T)(1.0 / 1920.0);
-    w[3] = ((T)841 + x * ((T)770 + x * ((T)-440 + x * ((T)-560 + x * ((T)80 + x * (T)160))))) * (T)(1.0 / 1920.0);
-    w[4] = ((T)237 + x * ((T)750 + x * ((T)840 + x * ((T)240 + x * ((T)-240 + x * (T)-160))))) * (T)(1.0 / 3840.0);
-    w[5] = ((T)1 + x * ((T)10 + x * ((T)40 + x * ((T)80 + x * ((T)80 + x * (T)32))))) * (T)(1.0 / 3840.0);
+    w[0] = w6_0<T>(x);
+    w[1] = w6_1<T>(x);
+    w[2] = w6_2<T>(x);
+    w[3] = w6_2<T>(-x);
+    w[4] = w6_1<T>(-x);
+    w[5] = w6_0<T>(-x);
 }
 
-// Base index floor(x/h) and the weights of the offset x/h - (ix + 1/2)
-// (yukawa_pm_ord6.py:17-27 takes x/h once and subtracts the same way); the
-// index and offset are formed in FP64 from the FP64 position, with x/h as
-// x * (1/h): one multiply instead of an FP64 division (each of a particle's
-// six lanes forms all three axes).  The two differ by an ulp of x/h; at a
-// node boundary that can move ix by one with the offset at +-1/2, where the
-// order-6 weights of the outermost node are zero, so the deposit agrees to
-// rounding either way.
+// Base index floor(x/h) and the offset x/h - (ix + 1/2) (yukawa_pm_ord6.py:17-27
+// takes x/h once and subtracts the same way); the index and offset are formed
+// in FP64 from the FP64 position, with x/h as x * (1/h): one multiply instead
+// of an FP64 division.  The two differ by an ulp of x/h; at a node boundary
+// that can move ix by one with the offset at +-1/2, where the order-6 weights
+// of the outermost node are zero, so the deposit agrees to rounding either way.
 template <class T>
-__device__ __forceinline__ int base_and_weights(double x, double ih, T w[6]) {
+__device__ __forceinline__ int base_and_offset(double x, double ih, T &off) {
     const double s = x * ih;
     const int i0 = (int)floor(s);
-    const double fr = s - (i0 + 0.5);
-    weights6<T>((T)fr, w);
+    off = (T)(s - (i0 + 0.5));
     return i0;
 }
 
@@ -62,7 +74,60 @@ __device__ __forceinline__ int wrap3(int i, int M) {
     return i;
 }
 
-constexpr int kLanesPerParticle = 8;  // six used: one stencil z-plane each
+// Gather: a particle's stencil is split over four consecutive lanes: lane
+// bit 0 picks the y rows (0, 1, 2 or 5, 4, 3), bit 1 the z planes (0, 2, 4 or
+// 5, 3, 1).  By the mirror identity above the weights of a lane's rows are
+// w0, w1, w2 at +-y and of its planes w0, w2 at +-z and w1 at -+z: every lane
+// evaluates the same polynomials, with no lane-dependent selects.
+constexpr int kLanesPerParticle = 4;
+
+template <class T>
+struct Quarter {
+    int ix, iy, iz;     // base node per axis
+    T wx[6], wy[3], wz[3];
+    int yo[3], zo[3];   // this lane's rows and planes within the stencil
+};
+
+template <class T>
+__device__ __forceinline__ void quarter(const Geom &g, const double4 &p, int sub, Quarter<T> &q) {
+    T fx, fy, fz;
+    q.ix = base_and_offset<T>(p.x, g.ih[0], fx);
+    q.iy = base_and_offset<T>(p.y, g.ih[1], fy);
+    q.iz = base_and_offset<T>(p.z, g.ih[2], fz);
+    const int yh = sub & 1, zh = sub >> 1;
+    weights6<T>(fx, q.wx);
+    const T sy = yh ? -fy : fy, sz = zh ? -fz : fz;
+    q.wy[0] = w6_0<T>(sy);
+    q.wy[1] = w6_1<T>(sy);
+    q.wy[2] = w6_2<T>(sy);
+    q.wz[0] = w6_0<T>(sz);
+    q.wz[1] = w6_2<T>(sz);
+    q.wz[2] = w6_1<T>(-sz);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        q.yo[b] = yh ? 5 - b : b;
+        q.zo[b] = zh ? 5 - 2 * b : 2 * b;
+    }
+}
+
+// Sum over each group of four lanes (all lanes active), by DPP quad permutes.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <class T>
+__device__ __forceinline__ T quad_sum(T v) {
+    v += dpp_mov<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+    v += dpp_mov<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+    return v;
+}
 
 // A PM block is CB x CB x CB reference cells (fewer at the upper box faces).
 // Its particles are 4 CB^2 (or fewer) contiguous runs of the sorted array:
@@ -80,8 +145,10 @@ __device__ __forceinline__ PMBlock pm_block(const Geom &g, int cb, int bid) {
     for (int d = 0; d < 3; ++d) {
         b.c0[d] = id[d] * cb;
         b.nc[d] = min(cb, g.nblk[d] - b.c0[d]);
-        // one node of margin below the lowest base index of the block's particles
-        b.o[d] = (int)floor((2 * b.c0[d]) * g.w[d] / g.h[d]) - 1;
+        // the lowest base index of the block's particles; one whose x/h
+        // rounds below it (within an ulp of the block face) is outside the
+        // brick and takes the kernels' global-memory path
+        b.o[d] = (int)floor((2 * b.c0[d]) * g.w[d] / g.h[d]);
     }
     return b;
 }
@@ -147,14 +214,21 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     for (int t = threadIdx.x; t < nb; t += blockDim.x) brick[t] = 0.0;
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
     const int npart = run_off[nrun];
-    const int sub = threadIdx.x % kLanesPerParticle;  // stencil z-plane of this lane
-    for (int k = threadIdx.x / kLanesPerParticle; k < npart; k += blockDim.x / kLanesPerParticle) {
+    // eight lanes a particle, six of them one stencil z-plane each: with half
+    // as many particles per wave as the four-lane split of the gather, fewer
+    // LDS atomics of one instruction land on the same node (measured ~6%
+    // faster here; the gather, with plain reads, gains from the denser split)
+    const int sub = threadIdx.x % 8;
+    for (int k = threadIdx.x / 8; k < npart; k += blockDim.x / 8) {
         if (sub >= 6) continue;
         const double4 p = P.spos[pm_particle(run_start, run_off, nrun, k)];
-        T wx[6], wy[6], wz[6];
-        const int ix = base_and_weights<T>(p.x, g.ih[0], wx);
-        const int iy = base_and_weights<T>(p.y, g.ih[1], wy);
-        const int iz = base_and_weights<T>(p.z, g.ih[2], wz);
+        T fx, fy, fz, wx[6], wy[6], wz[6];
+        const int ix = base_and_offset<T>(p.x, g.ih[0], fx);
+        const int iy = base_and_offset<T>(p.y, g.ih[1], fy);
+        const int iz = base_and_offset<T>(p.z, g.ih[2], fz);
+        weights6<T>(fx, wx);
+        weights6<T>(fy, wy);
+        weights6<T>(fz, wz);
         const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
         const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
         T wza = wz[0];
@@ -285,7 +359,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
     const int npart = run_off[nrun];
     double ksum = 0.0;
-    const int sub = threadIdx.x % kLanesPerParticle;  // stencil z-plane of this lane
+    const int sub = threadIdx.x % kLanesPerParticle;  // stencil quarter of this lane
     const int per_pass = blockDim.x / kLanesPerParticle;
     const int npass = (npart + per_pass - 1) / per_pass;
     for (int pass = 0; pass < npass; ++pass) {
@@ -296,40 +370,40 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
         double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
         if (live) {
             p = P.spos[s];
-            if (sub < 6) {
-                T wx[6], wy[6], wz[6];
-                const int ix = base_and_weights<T>(p.x, g.ih[0], wx);
-                const int iy = base_and_weights<T>(p.y, g.ih[1], wy);
-                const int iz = base_and_weights<T>(p.z, g.ih[2], wz);
-                T wza = wz[0];
+            Quarter<T> q;
+            quarter<T>(g, p, sub, q);
+            const int rx = q.ix - o0, ry = q.iy - o1, rz = q.iz - o2;
+            const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+            if (LDS && inside) {
 #pragma unroll
-                for (int a = 1; a < 6; ++a)
-                    if (sub == a) wza = wz[a];
-                const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
-                const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
-                if (LDS && inside) {
+                for (int m = 0; m < 3; ++m) {
 #pragma unroll
-                    for (int b = 0; b < 6; ++b) {
-                        const T wzy = wza * wy[b];
-                        const T *row = ebrick + 3 * (((rz + sub) * B1 + (ry + b)) * B0 + rx);
+                    for (int b = 0; b < 3; ++b) {
+                        // FP64: loads are not hoisted across rows, which keeps them
+                        // within the 128-VGPR limit of the 1024-thread workgroup
+                        if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
+                        const T wzy = q.wz[m] * q.wy[b];
+                        const T *row = ebrick + 3 * (((rz + q.zo[m]) * B1 + (ry + q.yo[b])) * B0 + rx);
 #pragma unroll
                         for (int c = 0; c < 6; ++c) {
-                            const T ww = wzy * wx[c];
+                            const T ww = wzy * q.wx[c];
                             ex += row[3 * c + 0] * ww;
                             ey += row[3 * c + 1] * ww;
                             ez += row[3 * c + 2] * ww;
                         }
                     }
-                } else {
-                    const int gz = wrap3(iz + sub, g.M[2]);
-                    for (int b = 0; b < 6; ++b) {
-                        const int gy = wrap3(iy + b, g.M[1]);
-                        const T wzy = wza * wy[b];
+                }
+            } else {
+                for (int m = 0; m < 3; ++m) {
+                    const int gz = wrap3(q.iz + q.zo[m], g.M[2]);
+                    for (int b = 0; b < 3; ++b) {
+                        const int gy = wrap3(q.iy + q.yo[b], g.M[1]);
+                        const T wzy = q.wz[m] * q.wy[b];
                         const T *row = emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
 #pragma unroll
                         for (int c = 0; c < 6; ++c) {
-                            const int gx = wrap3(ix + c, g.M[0]);
-                            const T ww = wzy * wx[c];
+                            const int gx = wrap3(q.ix + c, g.M[0]);
+                            const T ww = wzy * q.wx[c];
                             ex += row[3 * gx + 0] * ww;
                             ey += row[3 * gx + 1] * ww;
                             ez += row[3 * gx + 2] * ww;
@@ -338,13 +412,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
                 }
             }
         }
-        // sum the six planes of each particle (groups of eight lanes)
-#pragma unroll
-        for (int o = kLanesPerParticle / 2; o > 0; o >>= 1) {
-            ex += __shfl_xor(ex, o, 64);
-            ey += __shfl_xor(ey, o, 64);
-            ez += __shfl_xor(ez, o, 64);
-        }
+        // sum the four quarters of each particle
+        ex = quad_sum(ex);
+        ey = quad_sum(ey);
+        ez = quad_sum(ez);
         if (!live || sub != 0) continue;
         const int i = P.perm[s];
         const double apx = p.w * (double)ex, apy = p.w * (double)ey, apz = p.w * (double)ez;

@@ -293,6 +293,56 @@ def test_edge_positions_wrap_like_periodic_images(P3M):
     e.close()
 
 
+@pytest.fixture(scope="module")
+def faces_case():
+    """Coordinates exactly on reference-cell faces (which are mesh nodes here:
+    4 cells of 16 nodes per axis), one ulp either side of them, and on other
+    mesh nodes, with the oracle's result."""
+    N, M = 10000, 64
+    L = O.box_length(N)
+    rc = L / 4.2  # floor(L / rc) = 4 reference cells of L/4 = 16 h per axis
+    P = O.P3MParams(N, L, 1.5, 0.7, rc, M)
+    rng = np.random.default_rng(31)
+    pos = rng.uniform(0.0, L, (N, 3))
+    face = (L / 4) * rng.integers(0, 4, N)
+    node = (L / M) * rng.integers(0, M, N)
+    ax = rng.integers(0, 3, N)
+    kind = rng.integers(0, 5, N)
+    rows = np.arange(N)
+    pos[rows[kind == 0], ax[kind == 0]] = face[kind == 0]
+    pos[rows[kind == 1], ax[kind == 1]] = np.nextafter(face[kind == 1], -np.inf)
+    pos[rows[kind == 2], ax[kind == 2]] = np.nextafter(face[kind == 2], np.inf)
+    pos[rows[kind == 3], ax[kind == 3]] = node[kind == 3]
+    pos = np.where(pos < 0, pos + L, pos)
+    pos = np.where(pos >= L, pos - L, pos)
+    U_ref, acc_ref, parts = P.force(pos)
+    return dict(P=P, pos=pos, U=U_ref, acc=acc_ref, rho=parts["rho_r"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_positions_on_cell_faces_and_mesh_nodes(P3M, faces_case, precision):
+    """x/h and the cell index round at their edges on these coordinates, and
+    a charge stencil can start one node outside its block's brick (it then
+    takes the global-memory path).  Charge mesh, forces and energy match the
+    oracle to the plain or mixed bar."""
+    P, pos = faces_case["P"], faces_case["pos"]
+    e = P3M(P.N, P.M, P.L, P.kappa, P.G, P.rc, precision=precision)
+    e.set_green(dev(P.G_k))
+    rho = torch.empty((P.M, P.M, P.M), dtype=torch.float64, device="cuda")
+    acc, en = e.force(dev(pos), None, rho=rho)
+    torch.cuda.synchronize()
+    err = rel_rms(acc.cpu().numpy(), faces_case["acc"])
+    print(f"[faces] {precision}: rel rms {err:.3g}, U {float(en[0])} vs {faces_case['U']}, {e.stats()}")
+    tol = TOL if precision == "fp64" else 2e-6
+    assert err <= tol
+    assert float(en[0]) == pytest.approx(faces_case["U"], rel=tol)
+    # mixed: FP32 stencil weights (~1e-7 relative each), FP64 accumulation
+    rho_ref = faces_case["rho"]
+    atol = 1e-13 if precision == "fp64" else 1e-6 * float(np.abs(rho_ref).max())
+    np.testing.assert_allclose(rho.cpu().numpy(), rho_ref, atol=atol)
+    e.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_positions_several_periods_out(P3M, precision):
     """Ions displaced by whole periods (up to +-5 L, one by 1000 L), as a
