@@ -287,7 +287,12 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
         hoff[h + 1] = hoff[h] + (cum[idx + 1] - cum[idx]);
     }
     const int n_home = hoff[8];
-    int *q = queue + (wave * kGroups + grp) * kQCap;
+    int *const qw = queue + __builtin_amdgcn_readfirstlane(wave) * (kGroups * kQCap);  // the wave's two queues
+    int *q = qw + grp * kQCap;                                                           // this half's queue
+    // all ones in the upper half; opaque to the compiler, which would
+    // otherwise turn gmask & x into a select on grp (that needs x in a VGPR)
+    unsigned gmask = grp ? ~0u : 0u;
+    asm volatile("" : "+v"(gmask));
     const R rc2m = P.rc2m;
     double u_acc = 0.0;
 
@@ -468,17 +473,28 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         ddz = zi - tz[jc];
                     }
                     const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const bool c1 = r2 < rc2m, c2 = kRel ? (j != islot) : (r2 > (R)0);
+                    // FP32 has no self test here: i's own slot gives r^2 = 0
+                    // exactly from the hi + lo parts, which the pair drops
+                    const bool c1 = r2 < rc2m, c2 = kRel || r2 > (R)0;
                     const bool hit = inrow & c1 & c2;
                     // the mask as the AND of the compares' own masks: no VALU to materialise it
-                    const unsigned long long m = __builtin_amdgcn_ballot_w64(inrow) & __builtin_amdgcn_ballot_w64(c1) &
-                                                 __builtin_amdgcn_ballot_w64(c2);
+                    unsigned long long m = __builtin_amdgcn_ballot_w64(inrow) & __builtin_amdgcn_ballot_w64(c1);
+                    if constexpr (!kRel) m &= __builtin_amdgcn_ballot_w64(c2);
                     const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
                     const int clo = __popc(mlo), chi = __popc(mhi);
-                    // queue slot: the half's fill plus the hits on lower lanes of
-                    // the half (mbcnt counts all of the lower half for upper lanes)
-                    const int qb = grp ? qn1 - clo : qn0;
-                    if (hit) q[qb + __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u))] = j;
+                    // queue slot: the half's queue and fill plus the hits on lower
+                    // lanes of the half.  mbcnt counts all of the lower half for
+                    // upper lanes, so the upper half's slot is kQCap + qn1 - clo +
+                    // mbcnt.  As qn0 + (gmask & qd) + mbcnt, with qd wave-uniform
+                    // and qn0 folded into the wave's queue base, the per-half
+                    // select is one AND (a select of two SGPR values would take
+                    // two moves and a v_cndmask: one SGPR operand per VALU op)
+                    // (the difference of the fills through readfirstlane: not
+                    // reassociated with qn0 + clo, which would pull the fill
+                    // update into the divergent branch below)
+                    const unsigned qd = (unsigned)(__builtin_amdgcn_readfirstlane(kQCap + qn1 - qn0) - clo);
+                    const int slot = (int)__builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, gmask & qd));
+                    if (hit) qw[qn0 + slot] = j;
                     qn0 += clo;
                     qn1 += chi;
                     if (qn0 >= kGL || qn1 >= kGL) {
