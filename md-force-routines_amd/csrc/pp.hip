@@ -457,8 +457,8 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 const int lo = lr.x, hi = lr.y;
                 const int len = hi - lo;
                 const int nit = (max(readlane_i(len, 0), readlane_i(len, kGL)) + kGL - 1) / kGL;
-                for (int it = 0; it < nit; ++it) {
-                    const int j = lo + it * kGL + gl;
+                // one candidate batch: tile read and cutoff test, the hit mask in SGPRs
+                auto test = [&](int j, bool &hit, unsigned long long &m) {
                     const bool inrow = j < hi;
                     const int jc = inrow ? j : 0;  // slot 0 is always inside the LDS tile
                     R ddx, ddy, ddz;
@@ -476,10 +476,13 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                     // FP32 has no self test here: i's own slot gives r^2 = 0
                     // exactly from the hi + lo parts, which the pair drops
                     const bool c1 = r2 < rc2m, c2 = kRel || r2 > (R)0;
-                    const bool hit = inrow & c1 & c2;
+                    hit = inrow & c1 & c2;
                     // the mask as the AND of the compares' own masks: no VALU to materialise it
-                    unsigned long long m = __builtin_amdgcn_ballot_w64(inrow) & __builtin_amdgcn_ballot_w64(c1);
+                    m = __builtin_amdgcn_ballot_w64(inrow) & __builtin_amdgcn_ballot_w64(c1);
                     if constexpr (!kRel) m &= __builtin_amdgcn_ballot_w64(c2);
+                };
+                // queue the batch's hits; evaluate 32 pairs of a half whose queue filled
+                auto enqueue = [&](int j, bool hit, unsigned long long m) {
                     const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
                     const int clo = __popc(mlo), chi = __popc(mhi);
                     // queue slot: the half's queue and fill plus the hits on lower
@@ -510,6 +513,19 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         qn0 = max(qn0 - kGL, 0);
                         qn1 = max(qn1 - kGL, 0);
                     }
+                };
+                // two batches a trip, both tested before either is queued, so
+                // their LDS reads and arithmetic overlap.  On the last trip of an
+                // odd count the second batch starts at lo + nit * kGL >= hi of
+                // both halves and has no hits.
+                for (int it = 0; it < nit; it += 2) {
+                    const int j0 = lo + it * kGL + gl, j1 = j0 + kGL;
+                    bool h0, h1;
+                    unsigned long long m0, m1;
+                    test(j0, h0, m0);
+                    test(j1, h1, m1);
+                    enqueue(j0, h0, m0);
+                    enqueue(j1, h1, m1);
                 }
             }
             const int qmine = grp ? qn1 : qn0;
