@@ -176,6 +176,9 @@ __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, doubl
     }
 }
 
+// image shift of tile sub-cell index c (one period at most) along an axis of n sub-cells
+__device__ __forceinline__ double image_shift(int c, int n, double L) { return c < 0 ? -L : (c >= n ? L : 0.0); }
+
 __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 // Sum of v over each 32-lane half of the wave, returned in every lane of the
@@ -221,7 +224,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     R2 *tab = kRel ? (kPack ? (R2 *)(tl2 + T) : (R2 *)(tl4 + T)) : (R2 *)(tz + T);  // npiece * (D + 1)
     int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // NW * kGroups * kQCap
-    __shared__ int tcell[216];
+    __shared__ int tstart[216];  // sorted-array start of each tile sub-cell
     __shared__ int2 rng[8][5];   // per home sub-cell: LDS range of each of its 5 stencil planes
     __shared__ double red[NW];
 
@@ -247,8 +250,9 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
         wrap_cell(2 * by - 2 + tys, g.nsub[1], g.L[1], cy, sdummy);
         wrap_cell(2 * bz - 2 + tzs, g.nsub[2], g.L[2], cz, sdummy);
         const int cell = cx + g.nsub[0] * (cy + g.nsub[1] * cz);
-        tcell[tid] = cell;
-        cum[tid + 1] = P.cell_start[cell + 1] - P.cell_start[cell];
+        const int s0 = P.cell_start[cell];
+        tstart[tid] = s0;
+        cum[tid + 1] = P.cell_start[cell + 1] - s0;
     }
     __syncthreads();
     // 2. exclusive prefix over the 216 counts (wave 0: 4 entries per lane)
@@ -310,26 +314,17 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
         // 3. stage rows [r0, r1) with their image shifts
         for (int r = r0 + wave; r < r1; r += NW) {
             const int tys = r % 6, tzs = r / 6;
-            int cy, cz;
-            double sy, sz;
-            wrap_cell(2 * by - 2 + tys, g.nsub[1], g.L[1], cy, sy);
-            wrap_cell(2 * bz - 2 + tzs, g.nsub[2], g.L[2], cz, sz);
+            const double sy = image_shift(2 * by - 2 + tys, g.nsub[1], g.L[1]);
+            const double sz = image_shift(2 * bz - 2 + tzs, g.nsub[2], g.L[2]);
             int txs = 0;
             while (txs < 6) {
-                int cx0;
-                double sx;
-                wrap_cell(2 * bx - 2 + txs, g.nsub[0], g.L[0], cx0, sx);
+                const double sx = image_shift(2 * bx - 2 + txs, g.nsub[0], g.L[0]);
                 int txe = txs + 1;
-                while (txe < 6) {
-                    int cxe;
-                    double sxe;
-                    wrap_cell(2 * bx - 2 + txe, g.nsub[0], g.L[0], cxe, sxe);
-                    if (sxe != sx) break;
-                    ++txe;
-                }
-                const int c0 = cx0 + g.nsub[0] * (cy + g.nsub[1] * cz);
-                const int gs = P.cell_start[c0];
-                const int ge = P.cell_start[c0 + (txe - txs)];
+                while (txe < 6 && image_shift(2 * bx - 2 + txe, g.nsub[0], g.L[0]) == sx) ++txe;
+                // sub-cells txs .. txe-1 of the row are consecutive cells, so
+                // consecutive in the sorted array: start and length from LDS
+                const int gs = tstart[6 * r + txs];
+                const int ge = gs + (cum[6 * r + txe] - cum[6 * r + txs]);
                 const int dst = cum[6 * r + txs] - base;
                 for (int t = lane; t < ge - gs; t += 64) {
                     const double4 p = P.spos[gs + t];
@@ -386,7 +381,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 }
             const int hx = h & 1, hy = (h >> 1) & 1, hz = h >> 2;
             const int hidx = (2 + hx) + 6 * ((2 + hy) + 6 * (2 + hz));
-            const int gi = P.cell_start[tcell[hidx]] + (kk - hb);
+            const int gi = tstart[hidx] + (kk - hb);  // from LDS: no global load per i
             // tile slot of i in this pass (its row may be outside the pass)
             const int irow = (2 + hy) + 6 * (2 + hz);
             const int islot = (irow >= r0 && irow < r1) ? cum[hidx] - base + (kk - hb) : -1;
