@@ -94,16 +94,16 @@ constexpr int kGL = 64 / kGroups;      // lanes per i
 constexpr int kQCap = 2 * kGL;         // queue entries per group
 
 // Clenshaw sums of both series of one piece at t in [-1, 1].
-template <class T, class T2>
-__device__ __forceinline__ void cheb2(const T2 *__restrict__ c, T t, T &g1, T &g2) {
+template <class T, class C>
+__device__ __forceinline__ void cheb2(C c, T t, T &g1, T &g2) {
     constexpr int D = ChebDeg<T>::v;
     const T t2 = t + t;
     T a1 = 0, a2 = 0, b1 = 0, b2 = 0;
 #pragma unroll
     for (int k = D; k >= 1; --k) {
-        const T2 ck = c[k];
-        const T na = fma(t2, a1, ck.x - a2);
-        const T nb = fma(t2, b1, ck.y - b2);
+        const T cx = c[k].x, cy = c[k].y;
+        const T na = fma(t2, a1, cx - a2);
+        const T nb = fma(t2, b1, cy - b2);
         a2 = a1;
         a1 = na;
         b2 = b1;
@@ -114,15 +114,14 @@ __device__ __forceinline__ void cheb2(const T2 *__restrict__ c, T t, T &g1, T &g
 }
 
 // Horner sums of both power series of one piece at t in [-1, 1] (FP32 tables).
-template <class T, class T2>
-__device__ __forceinline__ void horner2(const T2 *__restrict__ c, T t, T &g1, T &g2) {
+template <class T, class C>
+__device__ __forceinline__ void horner2(C c, T t, T &g1, T &g2) {
     constexpr int D = ChebDeg<T>::v;
     T a = c[D].x, b = c[D].y;
 #pragma unroll
     for (int k = D - 1; k >= 0; --k) {
-        const T2 ck = c[k];
-        a = fma(a, t, ck.x);
-        b = fma(b, t, ck.y);
+        a = fma(a, t, (T)c[k].x);
+        b = fma(b, t, (T)c[k].y);
     }
     g1 = a;
     g2 = b;
@@ -150,12 +149,15 @@ __device__ __forceinline__ void pp_pair_cheb(const PPArgs<T> &P, const typename 
     if (sizeof(T) == 4 && !(r2 > (T)0)) return;
     const T ir = rsqrt_t<T>(r2);
     const T r = r2 * ir;
-    int p = (int)(r * P.inv_piece_w);
-    p = p < P.npiece - 1 ? (p > 0 ? p : 0) : P.npiece - 1;
+    // r > 0, so p >= 0: only the top needs a clamp (r within rounding of rc)
+    const int p = min((int)(r * P.inv_piece_w), P.npiece - 1);
     const T t = (r - p * P.piece_w) * ((T)2 * P.inv_piece_w) - (T)1;
+    // the table through an LDS-typed pointer: 32-bit address arithmetic
+    using LdsT2 = __attribute__((address_space(3))) const typename Vec2<T>::t;
+    LdsT2 *c = (LdsT2 *)tab + __umul24((unsigned)p, D + 1);  // p < 2^24: a full-rate 24-bit multiply
     T g1, g2;
-    if constexpr (sizeof(T) == 4) horner2<T>(tab + p * (D + 1), t, g1, g2);  // power basis (FP32 table)
-    else cheb2<T>(tab + p * (D + 1), t, g1, g2);
+    if constexpr (sizeof(T) == 4) horner2<T>(c, t, g1, g2);  // power basis (FP32 table)
+    else cheb2<T>(c, t, g1, g2);
     u += g1 * ir;
     const T fr = g2 * ir * ir * ir;
     fx += fr * dx;
