@@ -26,12 +26,13 @@
 // * Sweep.  A wave handles two i at a time, one per 32-lane half.  Each half
 //   sweeps the 5 planes of its i's 5x5x5 sub-cell stencil, each plane one
 //   contiguous LDS range (its 5 rows are consecutive tile rows; tabulated
-//   once per home sub-cell), testing 32 candidates per step; in-cutoff j
-//   are compacted into the half's LDS queue with a ballot and mbcnt, and the
+//   once per home sub-cell), testing two batches of 32 candidates per step;
+//   in-cutoff j are compacted into the half's LDS queue with a ballot and mbcnt, and the
 //   pair terms run only on queue batches, so lanes stay busy despite the
 //   ~25% in-cutoff rate of a cell stencil.
 // * Each pair is evaluated from both ends (no Newton-3 scatter, no atomics);
-//   i's force and energy are summed across its half-wave with shuffles.
+//   i's force and energy are summed across its half-wave (DPP moves in
+//   FP32, shuffles in FP64).
 //   The energy is halved at the end: U_s_r = 1/2 sum_i sum_{j != i} phi.
 // * A tile larger than the LDS capacity is processed in several passes over
 //   its rows (i's force accumulates in global memory across passes).
@@ -42,10 +43,11 @@
 //   hi difference of two nearby coordinates is exact (Sterbenz), so the
 //   displacement keeps FP32 relative accuracy even for pairs far closer than
 //   the hi rounding (2^-24 of 1.5 cells).  The pair terms use FP32 tables
-//   (degree kChebDegF, fitted to 1e-7 of max|g|) and FP32 arithmetic; each
-//   lane's few pair sums are widened to FP64 before the half-wave and block
-//   reductions.  Self pairs are excluded by tile slot, coincident particles
-//   (zero displacement) by the pair test.
+//   (degree kChebDegF, fitted to 1e-7 of max|g|, evaluated in the power
+//   basis by Horner) and FP32 arithmetic; i's FP32 half-wave sums are
+//   widened to FP64 for its acc write and the block energy.  The self pair
+//   (hi and lo identical: zero displacement) and coincident particles are
+//   dropped by the pair test.
 #include "mdf_internal.h"
 
 namespace mdf {
