@@ -210,9 +210,9 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     }
     // waves per PP workgroup.  FP64 (95 VGPR): 16 for tiles above 64 KB (one
     // workgroup per CU), else 4.  FP32 (71 VGPR; counted as at most 6 waves a
-    // SIMD = 24 a CU, the most that workgroups of 4k waves reach: a 14-wave
-    // workgroup splits 4,4,3,3 over the SIMDs and a second one does not fit
-    // beside it): the count of 8, 12 or 16 that puts the most waves on a CU given the
+    // SIMD = 24 a CU: a 14-wave kernel aiming at 28 ran 1.5x slower, as if
+    // its 4,4,3,3 split over the SIMDs kept a second workgroup out): the
+    // count of 8, 12 or 16 that puts the most waves on a CU given the
     // LDS of each workgroup (dynamic + ~1 KB static of 160 KB), the fewest
     // waves on a tie; 16 when even one workgroup's tile overflows the LDS
     // (multipass).  The 12-wave kernel packs its tile to 24 B a particle
