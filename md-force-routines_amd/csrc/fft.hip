@@ -425,7 +425,9 @@ static FftPlan<T> fft_plan(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const t
     F.nl = nl;
     F.g1 = (My * Mz) / (2 * nl);
     F.lds1 = 2 * cs * nl * Mx;
-    F.kty = My >= 512 ? 4 : (My >= 256 ? 8 : 16);
+    // columns a block: 8 below 256 (16 measured 2-3 us slower at 128^3:
+    // half as many blocks, each a longer chain of barrier-separated stages)
+    F.kty = My >= 512 ? 4 : 8;
     F.g2x = (Mxh + F.kty - 1) / F.kty;
     F.lds2 = 2 * cs * F.kty * My;
     GreenZArgs<T> &P = F.gz;
@@ -441,7 +443,7 @@ static FftPlan<T> fft_plan(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const t
     P.Mxh = Mxh;
     P.lm = F.lz;
     P.kt = Mz >= 512 ? 2 : (Mz >= 256 ? 4 : 8);
-    if (sizeof(T) == 4) P.kt *= 2;
+    if (sizeof(T) == 4 && Mz >= 256) P.kt *= 2;  // FP32 at 128^3: 8 columns took the pass 6 us below 16
     for (int d = 0; d < 3; ++d) P.kfac[d] = 2.0 * M_PI / c->g.L[d];
     P.inv_V = 1.0 / (c->g.L[0] * c->g.L[1] * c->g.L[2]);
     F.g3x = (Mxh + P.kt - 1) / P.kt;
