@@ -79,7 +79,8 @@ __device__ __forceinline__ int wrap3(int i, int M) {
 // 5, 3, 1).  By the mirror identity above the weights of a lane's rows are
 // w0, w1, w2 at +-y and of its planes w0, w2 at +-z and w1 at -+z: every lane
 // evaluates the same polynomials, with no lane-dependent selects.
-constexpr int kLanesPerParticle = 4;
+constexpr int kGatherLanes = 4;   // lanes a particle in the gather
+constexpr int kScatterLanes = 8;  // in the assignment (six busy: one z-plane each)
 
 template <class T>
 struct Quarter {
@@ -218,8 +219,8 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     // as many particles per wave as the four-lane split of the gather, fewer
     // LDS atomics of one instruction land on the same node (measured ~6%
     // faster here; the gather, with plain reads, gains from the denser split)
-    const int sub = threadIdx.x % 8;
-    for (int k = threadIdx.x / 8; k < npart; k += blockDim.x / 8) {
+    const int sub = threadIdx.x % kScatterLanes;
+    for (int k = threadIdx.x / kScatterLanes; k < npart; k += blockDim.x / kScatterLanes) {
         if (sub >= 6) continue;
         const double4 p = P.spos[pm_particle(run_start, run_off, nrun, k)];
         T fx, fy, fz, wx[6], wy[6], wz[6];
@@ -359,11 +360,11 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
     const int npart = run_off[nrun];
     double ksum = 0.0;
-    const int sub = threadIdx.x % kLanesPerParticle;  // stencil quarter of this lane
-    const int per_pass = blockDim.x / kLanesPerParticle;
+    const int sub = threadIdx.x % kGatherLanes;  // stencil quarter of this lane
+    const int per_pass = blockDim.x / kGatherLanes;
     const int npass = (npart + per_pass - 1) / per_pass;
     for (int pass = 0; pass < npass; ++pass) {
-        const int k = pass * per_pass + threadIdx.x / kLanesPerParticle;
+        const int k = pass * per_pass + threadIdx.x / kGatherLanes;
         const bool live = k < npart;
         const int s = live ? pm_particle(run_start, run_off, nrun, k) : 0;
         T ex = 0, ey = 0, ez = 0;
