@@ -165,6 +165,26 @@ struct ModesArgs {
     double *part;
 };
 
+// Sum over the wave with DPP row moves (no LDS): quad permutes, the half-row
+// and row mirrors, then row_bcast:15 and row_bcast:31; the total ends in lane
+// 63 (the other lanes hold partial sums).  All lanes must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_last(double v) {
+    v = dpp_add_d<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add_d<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add_d<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_add_d<0x140, 0xF>(v);  // row_mirror
+    v = dpp_add_d<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add_d<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 __device__ __forceinline__ void modes_grid_body(const ModesArgs &A, const Blk &b) {
     // b.y: axis, b.z: chunk of kModeChunk modes starting at m0
     const double *__restrict__ pos = A.pos;
@@ -205,10 +225,10 @@ __device__ __forceinline__ void modes_grid_body(const ModesArgs &A, const Blk &b
     }
 #pragma unroll
     for (int k = 0; k < kModeChunk; ++k) {
-        const double sr = wave_sum(ar[k]);
-        const double si = wave_sum(ai[k]);
+        const double sr = wave_sum_last(ar[k]);
+        const double si = wave_sum_last(ai[k]);
         const int m = m0 + k;
-        if (lane == 0 && m < nq) {
+        if (lane == 63 && m < nq) {
             part[((size_t)m * nw + gw) * 6 + 2 * a] = sr;
             part[((size_t)m * nw + gw) * 6 + 2 * a + 1] = si;
         }
