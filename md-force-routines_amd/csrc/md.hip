@@ -300,11 +300,26 @@ struct FinalArgs {
     int rec_stride;
 };
 
+// Sum of x[0..n) over the block's threads, four independent chains a thread so
+// that its loads are in flight together (the order is fixed: deterministic).
+__device__ __forceinline__ double thread_sum4(const double *__restrict__ x, int n) {
+    const int st = blockDim.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        s0 += x[i];
+        s1 += x[i + st];
+        s2 += x[i + 2 * st];
+        s3 += x[i + 3 * st];
+    }
+    for (; i < n; i += st) s0 += x[i];
+    return (s0 + s1) + (s2 + s3);
+}
+
 __device__ __forceinline__ void finalize_body(const FinalArgs &P, int step, double *red) {
-    double a = 0.0, b = 0.0, k = 0.0;
-    for (int i = threadIdx.x; i < P.npp; i += blockDim.x) a += P.pp[i];
-    for (int i = threadIdx.x; i < P.npm; i += blockDim.x) b += P.pm[i];
-    for (int i = threadIdx.x; i < P.nk; i += blockDim.x) k += P.kp[i];
+    double a = thread_sum4(P.pp, P.npp);
+    double b = thread_sum4(P.pm, P.npm);
+    double k = thread_sum4(P.kp, P.nk);
     a = block_sum(a, red);
     b = block_sum(b, red);
     k = block_sum(k, red);
