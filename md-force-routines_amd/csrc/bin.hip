@@ -74,9 +74,10 @@ __global__ void k_bin_count_seg(const BinArgs *__restrict__ tab, SegGrid S) {
     bin_count_body(tab[r], b);
 }
 
-// Exclusive scan of the cell counts in three passes over tiles of
-// kScanTile cells: per-tile sums, a scan of the tile sums (one block), and a
-// per-tile scan that adds the tile offset, writes cell_start and resets the
+// Exclusive scan of the cell counts in two passes over tiles of kScanTile
+// cells: per-tile sums, then a per-tile scan that adds the sum of the earlier
+// tiles' sums (each block adds those up itself: a few hundred tiles at most
+// in practice, 8192 at the cell limit), writes cell_start and resets the
 // counts for the next binning.
 constexpr int kScanTile = 1024;  // 256 threads x 4 cells
 
@@ -117,58 +118,18 @@ __global__ __launch_bounds__(256) void k_scan_tile_sums_seg(const BinArgs *__res
     scan_tile_sums_body(tab[r], b, wsum);
 }
 
-// One block of 1024 threads scans up to 1024 * 8 tile sums in place (exclusive).
-__device__ __forceinline__ void scan_tiles_body(const BinArgs &A, int *wsum) {
-    int *__restrict__ tile_sum = A.tile;
-    const int ntile = A.ntile;
-    int *__restrict__ total = A.tile + A.ntile;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    constexpr int per = 8;
-    int v[per];
-    int s = 0;
-#pragma unroll
-    for (int q = 0; q < per; ++q) {
-        const int k = t * per + q;
-        v[q] = k < ntile ? tile_sum[k] : 0;
-        s += v[q];
-    }
-    int incl = s;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += u;
-    }
-    if (lane == 63) wsum[wv] = incl;
-    __syncthreads();
-    int add = 0;
-    for (int w = 0; w < wv; ++w) add += wsum[w];
-    int run = incl - s + add;
-#pragma unroll
-    for (int q = 0; q < per; ++q) {
-        const int k = t * per + q;
-        if (k < ntile) tile_sum[k] = run;
-        run += v[q];
-    }
-    if (t == 1023) *total = run;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_tiles(BinArgs A) {
-    __shared__ int wsum[16];
-    scan_tiles_body(A, wsum);
-}
-__global__ __launch_bounds__(1024) void k_scan_tiles_seg(const BinArgs *__restrict__ tab, SegGrid S) {
-    __shared__ int wsum[16];
-    Blk b;
-    const int r = seg_locate(S, b);
-    scan_tiles_body(tab[r], wsum);
-}
-
 __device__ __forceinline__ void scan_apply_body(const BinArgs &A, const Blk &b, int *wsum) {
     int *__restrict__ cell_count = A.cell_count;
     const int ncell = A.ncell;
-    const int *__restrict__ tile_off = A.tile;
-    const int *__restrict__ total = A.tile + A.ntile;
+    const int *__restrict__ tile_sum = A.tile;
     int *__restrict__ cell_start = A.cell_start;
+    // offset of this tile: the sum of the earlier tiles' sums
+    int part = 0;
+    for (int k = threadIdx.x; k < b.x; k += blockDim.x) part += tile_sum[k];
+    const int part_incl = block_incl_scan_256(part, wsum);
+    if (threadIdx.x == 255) wsum[4] = part_incl;
+    __syncthreads();
+    const int tile_off = wsum[4];
     const int base = b.x * kScanTile + threadIdx.x * 4;
     int v[4];
     int s = 0;
@@ -178,7 +139,7 @@ __device__ __forceinline__ void scan_apply_body(const BinArgs &A, const Blk &b, 
         s += v[q];
     }
     const int incl = block_incl_scan_256(s, wsum);
-    int run = tile_off[b.x] + incl - s;
+    int run = tile_off + incl - s;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (base + q < ncell) {
@@ -187,15 +148,16 @@ __device__ __forceinline__ void scan_apply_body(const BinArgs &A, const Blk &b, 
         }
         run += v[q];
     }
-    if (b.x == 0 && threadIdx.x == 0) cell_start[ncell] = *total;
+    // the last tile's last thread holds the total (cells past ncell count 0)
+    if (b.x == b.nx - 1 && threadIdx.x == 255) cell_start[ncell] = run;
 }
 
 __global__ __launch_bounds__(256) void k_scan_apply(BinArgs A) {
-    __shared__ int wsum[4];
+    __shared__ int wsum[5];
     scan_apply_body(A, blk_plain(), wsum);
 }
 __global__ __launch_bounds__(256) void k_scan_apply_seg(const BinArgs *__restrict__ tab, SegGrid S) {
-    __shared__ int wsum[4];
+    __shared__ int wsum[5];
     Blk b;
     const int r = seg_locate(S, b);
     scan_apply_body(tab[r], b, wsum);
@@ -284,8 +246,6 @@ int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hip
     MDF_LAUNCHED();
     hipLaunchKernelGGL(k_scan_tile_sums, dim3(A.ntile), dim3(256), 0, s, A);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, A);
-    MDF_LAUNCHED();
     hipLaunchKernelGGL(k_scan_apply, dim3(A.ntile), dim3(256), 0, s, A);
     MDF_LAUNCHED();
     hipLaunchKernelGGL(k_bin_place, dim3(nb), dim3(256), 0, s, A);
@@ -307,12 +267,6 @@ static int build_bin_tiles(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &d
     lds = 0;
     return MDF_OK;
 }
-static int build_bin_one(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
-    *(BinArgs *)args = bin_args(c, io.pos, nullptr, io.n);
-    dims = make_int4(1, 1, 1, 0);
-    lds = 0;
-    return MDF_OK;
-}
 #define MDF_SEG_LAUNCH(fname, kern, threads, Args)                                                        \
     static int fname(const void *tab, const SegGrid &S, int total, size_t lds, int k, hipStream_t s) {   \
         (void)lds;                                                                                          \
@@ -323,7 +277,6 @@ static int build_bin_one(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dim
     }
 MDF_SEG_LAUNCH(launch_bin_count, k_bin_count_seg, 256, BinArgs)
 MDF_SEG_LAUNCH(launch_scan_tile_sums, k_scan_tile_sums_seg, 256, BinArgs)
-MDF_SEG_LAUNCH(launch_scan_tiles, k_scan_tiles_seg, 1024, BinArgs)
 MDF_SEG_LAUNCH(launch_scan_apply, k_scan_apply_seg, 256, BinArgs)
 MDF_SEG_LAUNCH(launch_bin_place, k_bin_place_seg, 256, BinArgs)
 #undef MDF_SEG_LAUNCH
@@ -331,7 +284,6 @@ MDF_SEG_LAUNCH(launch_bin_place, k_bin_place_seg, 256, BinArgs)
 void seg_stages_bin(std::vector<SegStage> &out) {
     out.push_back({"bin_count", sizeof(BinArgs), build_bin_particles, launch_bin_count});
     out.push_back({"scan_tile_sums", sizeof(BinArgs), build_bin_tiles, launch_scan_tile_sums});
-    out.push_back({"scan_tiles", sizeof(BinArgs), build_bin_one, launch_scan_tiles});
     out.push_back({"scan_apply", sizeof(BinArgs), build_bin_tiles, launch_scan_apply});
     out.push_back({"bin_place", sizeof(BinArgs), build_bin_particles, launch_bin_place});
 }
