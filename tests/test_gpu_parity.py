@@ -344,6 +344,37 @@ def test_positions_on_cell_faces_and_mesh_nodes(P3M, faces_case, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_anisotropic_box_and_mesh(P3M, precision):
+    """Box 30 x 34 x 40 with a 32 x 32 x 64 mesh: the reference takes Lx, Ly, Lz
+    and Mx, My, Mz separately (yukawa_p3m_ord6.py:6).  3 x 4 x 5 reference
+    cells at rc 8.  Forces, U and the charge mesh against the oracle, and the
+    device Green function against gf_opt."""
+    N, M = 6000, (32, 32, 64)
+    Lv = np.array([30.0, 34.0, 40.0])
+    kappa, alpha, rc = 1.0, 0.6, 8.0
+    h = Lv / np.array(M)
+    G_k, kx_v, ky_v, kz_v, _ = O.gf_opt(kappa, alpha, 6, 3, 3, 3, *M, *h, *Lv)
+    pos = np.random.default_rng(41).uniform(0.0, 1.0, (N, 3)) * Lv
+    U_ref, acc_ref, parts = O.force_pot(kappa, alpha, N, rc, Lv, 1.0 / 3.0, 1.0, pos, np.ones(N), G_k, kx_v, ky_v,
+                                        kz_v, *M, *h, *Lv)
+    e = P3M(N, M, tuple(Lv), kappa, alpha, rc, precision=precision)
+    full, _ = e.build_green(6, 3, want_full=True)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(full.cpu().numpy() - G_k)) <= 1e-13 * np.max(np.abs(G_k))
+    rho = torch.empty((M[2], M[1], M[0]), dtype=torch.float64, device="cuda")
+    acc, en = e.force(dev(pos), None, af=1.0 / 3.0, uf=1.0, rho=rho)
+    torch.cuda.synchronize()
+    err = rel_rms(acc.cpu().numpy(), acc_ref)
+    print(f"[aniso] {precision}: rel rms {err:.3g}, U {float(en[0])} vs {U_ref}")
+    tol = TOL if precision == "fp64" else 2e-6
+    assert err <= tol
+    assert float(en[0]) == pytest.approx(U_ref, rel=tol)
+    atol = 1e-13 if precision == "fp64" else 1e-6 * float(np.abs(parts["rho_r"]).max())
+    np.testing.assert_allclose(rho.cpu().numpy(), parts["rho_r"], atol=atol)
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_positions_several_periods_out(P3M, precision):
     """Ions displaced by whole periods (up to +-5 L, one by 1000 L), as a
     drift faster than L/dt leaves them: the force is that of the in-box
