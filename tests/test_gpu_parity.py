@@ -375,6 +375,35 @@ def test_anisotropic_box_and_mesh(P3M, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nonunit_charges(P3M, precision):
+    """Charges Z uniform in [0.5, 1.5): the reference's PM weights the deposit
+    and the field by Z (yukawa_pm_ord6.py:52-93, :308-310) while its PP pair
+    terms are for unit charges (yukawa_pp.py:115-124).  Forces, U and the
+    charge mesh against the oracle."""
+    N, M = 4000, 32
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 2.0, 0.7, L / 3.5, M)
+    rng = np.random.default_rng(43)
+    pos = rng.uniform(0.0, L, (N, 3))
+    Z = rng.uniform(0.5, 1.5, N)
+    U_ref, acc_ref, parts = P.force(pos, Z)
+    e = P3M(N, M, L, P.kappa, P.G, P.rc, precision=precision)
+    e.set_green(dev(P.G_k))
+    rho = torch.empty((M, M, M), dtype=torch.float64, device="cuda")
+    acc, en = e.force(dev(pos), dev(Z), rho=rho)
+    torch.cuda.synchronize()
+    err = rel_rms(acc.cpu().numpy(), acc_ref)
+    print(f"[charges] {precision}: rel rms {err:.3g}, U {float(en[0])} vs {U_ref}")
+    tol = TOL if precision == "fp64" else 2e-6
+    assert err <= tol
+    assert float(en[0]) == pytest.approx(U_ref, rel=tol)
+    atol = 1e-13 if precision == "fp64" else 1e-6 * float(np.abs(parts["rho_r"]).max())
+    np.testing.assert_allclose(rho.cpu().numpy(), parts["rho_r"], atol=atol)
+    assert float(rho.sum()) == pytest.approx(float(Z.sum()), rel=1e-12 if precision == "fp64" else 1e-6)
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_positions_several_periods_out(P3M, precision):
     """Ions displaced by whole periods (up to +-5 L, one by 1000 L), as a
     drift faster than L/dt leaves them: the force is that of the in-box
