@@ -404,6 +404,31 @@ def test_nonunit_charges(P3M, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_coulomb_limit_kappa_zero(P3M, precision):
+    """kappa = 0 (the Coulomb limit, b = kappa / 2 alpha = 0 in the pair terms
+    and R(k) = 4 pi exp(-k^2 / 4 alpha^2) / k^2): full force, U and the device
+    Green function against the oracle."""
+    N, M = 3000, 32
+    L = O.box_length(N)
+    with np.errstate(divide="ignore", invalid="ignore"):  # gf_opt's k = 0 alias term, zeroed after (as the reference)
+        P = O.P3MParams(N, L, 0.0, 0.8, L / 3.2, M)
+    pos = np.random.default_rng(47).uniform(0.0, L, (N, 3))
+    U_ref, acc_ref, _ = P.force(pos)
+    e = P3M(N, M, L, 0.0, P.G, P.rc, precision=precision)
+    full, _ = e.build_green(6, 3, want_full=True)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(full.cpu().numpy() - P.G_k)) <= 1e-13 * np.max(np.abs(P.G_k))
+    acc, en = e.force(dev(pos))
+    torch.cuda.synchronize()
+    err = rel_rms(acc.cpu().numpy(), acc_ref)
+    print(f"[coulomb] {precision}: rel rms {err:.3g}, U {float(en[0])} vs {U_ref}")
+    tol = TOL if precision == "fp64" else 2e-6
+    assert err <= tol
+    assert float(en[0]) == pytest.approx(U_ref, rel=tol)
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_positions_several_periods_out(P3M, precision):
     """Ions displaced by whole periods (up to +-5 L, one by 1000 L), as a
     drift faster than L/dt leaves them: the force is that of the in-box
