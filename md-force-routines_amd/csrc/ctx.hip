@@ -249,17 +249,20 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         return MDF_EINVAL;
     }
 
-    // PM blocks: the largest cb (<= 4 reference cells per axis) whose brick
-    // fits the LDS budget; a brick spans ceil(cb * cell / h) + 6 nodes (the
-    // base-index range of the block and the 6-node stencil; a margin node
-    // below for rounding at the block face measured ~6% slower assignment,
-    // from the larger brick to clear and flush)
+    // PM blocks: the largest shape (up to 4 reference cells per axis) whose
+    // brick fits the LDS budget; a brick spans ceil(sb * w / h) + 6 nodes per
+    // axis for sb sub-cells of width w (the base-index range of the block and
+    // the 6-node stencil; a margin node below for rounding at the block face
+    // measured ~6% slower assignment, from the larger brick to clear and flush)
     {
-        auto extent = [&](int cb, int *B) {
+        // shapes in sub-cells per axis: whole cells (sb = 2 cb) for both
+        // kernels, then for the gather half a cell along x (and y)
+        static const int shapes[7][3] = {{8, 8, 8}, {6, 6, 6}, {4, 4, 4}, {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
+        auto extent = [&](const int *sb, int *B) {
             size_t nn = 1;
             for (int d = 0; d < 3; ++d) {
-                const int cbd = cb < g.nblk[d] ? cb : g.nblk[d];
-                B[d] = (int)ceil(cbd * 2.0 * g.w[d] / g.h[d]) + 6;
+                const int sbd = sb[d] < g.nsub[d] ? sb[d] : g.nsub[d];
+                B[d] = (int)ceil(sbd * g.w[d] / g.h[d]) + 6;
                 // odd x and y extents: the row and z-plane strides of the brick
                 // are then odd multiples of the element size, so the four lanes
                 // that work on the rows and planes of one particle spread over
@@ -269,36 +272,43 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             }
             return nn;
         };
-        auto nblocks = [&](int cb) {
-            return ((g.nblk[0] + cb - 1) / cb) * ((g.nblk[1] + cb - 1) / cb) * ((g.nblk[2] + cb - 1) / cb);
+        auto nblocks = [&](const int *sb) {
+            return ((g.nsub[0] + sb[0] - 1) / sb[0]) * ((g.nsub[1] + sb[1] - 1) / sb[1]) *
+                   ((g.nsub[2] + sb[2] - 1) / sb[2]);
         };
-        c->brick.cb_assign = 1;
-        c->brick.cb_gather = 1;
-        for (int cb = 4; cb >= 1; --cb) {
-            int B[3];
+        {
             // sized by node count (8192 nodes) in both precisions: a larger
-            // block would leave too few workgroups to fill the chip
-            if (extent(cb, B) * 8 <= 64 * 1024 || cb == 1) {
-                c->brick.cb_assign = cb;
-                for (int d = 0; d < 3; ++d) c->brick.B[d] = B[d];
-                c->brick.lds_assign = extent(cb, B) * 8 <= 128 * 1024 ? 1 : 0;
-                break;
-            }
-        }
-        for (int cb = 4; cb >= 1; --cb) {
+            // block would leave too few workgroups to fill the chip; whole
+            // cells only (2^3 sub-cells at the least)
+            int k = 0;
             int B[3];
+            while (k < 3 && extent(shapes[k], B) * 8 > 64 * 1024) ++k;
+            const size_t nn = extent(shapes[k], B);
+            for (int d = 0; d < 3; ++d) {
+                c->brick.sb_assign[d] = shapes[k][d];
+                c->brick.B[d] = B[d];
+            }
+            c->brick.lds_assign = nn * 8 <= 128 * 1024 ? 1 : 0;
+        }
+        {
             // 156 KB: with the gather's < 1 KB of static LDS within the 160 KB
             // of a CU (at 144 KB the FP64 brick of 2^3 cells at N = 1e6, rc 7,
-            // 152 KB, fell back to 1-cell blocks: 0.45 ms against 0.27 ms)
-            if (extent(cb, B) * 3 * es <= 156 * 1024 || cb == 1) {
-                c->brick.cb_gather = cb;
-                for (int d = 0; d < 3; ++d) c->brick.Bg[d] = B[d];
-                c->brick.lds_gather = extent(cb, B) * 3 * es <= 156 * 1024 ? 1 : 0;
-                break;
+            // 152 KB, fell back to 1-cell blocks: 0.45 ms against 0.27 ms).
+            // When no shape fits, 2^3 sub-cells read the global field meshes.
+            int k = 0;
+            int B[3];
+            while (k < 7 && extent(shapes[k], B) * 3 * es > 156 * 1024) ++k;
+            const bool fits = k < 7;
+            if (!fits) k = 3;
+            extent(shapes[k], B);
+            for (int d = 0; d < 3; ++d) {
+                c->brick.sb_gather[d] = shapes[k][d];
+                c->brick.Bg[d] = B[d];
             }
+            c->brick.lds_gather = fits ? 1 : 0;
         }
-        c->brick.nblock_assign = nblocks(c->brick.cb_assign);
-        c->brick.nblock_gather = nblocks(c->brick.cb_gather);
+        c->brick.nblock_assign = nblocks(c->brick.sb_assign);
+        c->brick.nblock_gather = nblocks(c->brick.sb_gather);
     }
 
     int r = MDF_OK;

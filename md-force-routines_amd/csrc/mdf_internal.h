@@ -55,12 +55,12 @@ struct Geom {
 };
 
 // Mesh bricks of the assignment and gather kernels.  Each workgroup handles
-// a PM block of cb^3 reference cells; its brick covers the stencils of the
-// block's particles.
+// a PM block of sb[0] x sb[1] x sb[2] sub-cells; its brick covers the
+// stencils of the block's particles.
 struct Brick {
     int B[3];          // assignment brick extent in mesh nodes
     int Bg[3];         // gather brick extent
-    int cb_assign, cb_gather;
+    int sb_assign[3], sb_gather[3];  // sub-cells a block per axis
     int nblock_assign, nblock_gather;
     int lds_assign;    // 1 if the charge brick fits LDS for the assignment
     int lds_gather;    // 1 if the 3-field brick fits LDS for the gather

@@ -130,26 +130,28 @@ __device__ __forceinline__ T quad_sum(T v) {
     return v;
 }
 
-// A PM block is CB x CB x CB reference cells (fewer at the upper box faces).
-// Its particles are 4 CB^2 (or fewer) contiguous runs of the sorted array:
+// A PM block is sb[0] x sb[1] x sb[2] sub-cells (fewer at the upper box
+// faces): whole reference cells (sb = 2 cb), or for the gather, whose FP64
+// brick of one cell can exceed the LDS, half a cell along x (and y).  Its
+// particles are sb[1] sb[2] (or fewer) contiguous runs of the sorted array:
 // one per row of sub-cells along y and z, spanning the block in x.
 struct PMBlock {
-    int c0[3], nc[3];  // first reference cell and cell count per axis
+    int s0[3], ns[3];  // first sub-cell and sub-cell count per axis
     int o[3];          // brick origin (mesh node) per axis
 };
 
-__device__ __forceinline__ PMBlock pm_block(const Geom &g, int cb, int bid) {
+__device__ __forceinline__ PMBlock pm_block(const Geom &g, const int *sb, int bid) {
     PMBlock b;
-    const int nbx = (g.nblk[0] + cb - 1) / cb, nby = (g.nblk[1] + cb - 1) / cb;
+    const int nbx = (g.nsub[0] + sb[0] - 1) / sb[0], nby = (g.nsub[1] + sb[1] - 1) / sb[1];
     const int id[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        b.c0[d] = id[d] * cb;
-        b.nc[d] = min(cb, g.nblk[d] - b.c0[d]);
+        b.s0[d] = id[d] * sb[d];
+        b.ns[d] = min(sb[d], g.nsub[d] - b.s0[d]);
         // the lowest base index of the block's particles; one whose x/h
         // rounds below it (within an ulp of the block face) is outside the
         // brick and takes the kernels' global-memory path
-        b.o[d] = (int)floor((2 * b.c0[d]) * g.w[d] / g.h[d]);
+        b.o[d] = (int)floor(b.s0[d] * g.w[d] / g.h[d]);
     }
     return b;
 }
@@ -158,16 +160,16 @@ __device__ __forceinline__ PMBlock pm_block(const Geom &g, int cb, int bid) {
 // returns the number of runs.  Call with all threads; ends with a barrier.
 __device__ int pm_runs(const Geom &g, const int *__restrict__ cell_start, const PMBlock &b, int *run_start,
                        int *run_off) {
-    const int ny = 2 * b.nc[1], nz = 2 * b.nc[2];
+    const int ny = b.ns[1], nz = b.ns[2];
     const int nrun = ny * nz;
     const int t = threadIdx.x;
     int len = 0;
     if (t < nrun) {
-        const int sy = 2 * b.c0[1] + t % ny, sz = 2 * b.c0[2] + t / ny;
-        const int cfirst = 2 * b.c0[0] + g.nsub[0] * (sy + g.nsub[1] * sz);
+        const int sy = b.s0[1] + t % ny, sz = b.s0[2] + t / ny;
+        const int cfirst = b.s0[0] + g.nsub[0] * (sy + g.nsub[1] * sz);
         const int s0 = cell_start[cfirst];
         run_start[t] = s0;
-        len = cell_start[cfirst + 2 * b.nc[0]] - s0;
+        len = cell_start[cfirst + b.ns[0]] - s0;
     }
     if (t < 64) {  // nrun <= 64: one wave scans the lengths
         int incl = len;
@@ -210,7 +212,7 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     const Geom &g = P.g;
     const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
     const int nb = B0 * B1 * B2;
-    const PMBlock blk = pm_block(g, P.br.cb_assign, bk.x);
+    const PMBlock blk = pm_block(g, P.br.sb_assign, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     for (int t = threadIdx.x; t < nb; t += blockDim.x) brick[t] = 0.0;
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
@@ -344,7 +346,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
     const T *emesh = (const T *)P.emesh;
     const Geom &g = P.g;
     const int B0 = P.br.Bg[0], B1 = P.br.Bg[1], B2 = P.br.Bg[2];
-    const PMBlock blk = pm_block(g, P.br.cb_gather, bk.x);
+    const PMBlock blk = pm_block(g, P.br.sb_gather, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     if (LDS) {
         const int nb = B0 * B1 * B2;
