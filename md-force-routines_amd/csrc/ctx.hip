@@ -208,8 +208,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             return rr;
         }
     }
-    // waves per PP workgroup.  FP64 (95 VGPR): 16 for tiles above 64 KB (one
-    // workgroup per CU), else 4.  FP32 (71 VGPR; counted as at most 6 waves a
+    // waves per PP workgroup.  FP64: below.  FP32 (71 VGPR; counted as at most 6 waves a
     // SIMD = 24 a CU: a 14-wave kernel aiming at 28 ran 1.5x slower, as if
     // its 4,4,3,3 split over the SIMDs kept a second workgroup out): the
     // count of 8, 12 or 16 that puts the most waves on a CU given the
@@ -230,7 +229,13 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         }
         c->pp_waves = best;
     } else {
-        c->pp_waves = pp_lds_bytes(c->tile_max, c->npiece, false, 4) > 64 * 1024 ? 16 : 4;
+        // FP64: 4 waves for tiles up to 64 KB (several workgroups a CU); above
+        // that two workgroups of 12 (80 VGPR, see k_pp) when their LDS fits,
+        // else one of 16.  At rc 5.5 the 4-wave layout beat 12 waves with
+        // the spill (1.69 against 1.78 ms), so 12 replaces 16 only.
+        const bool big = pp_lds_bytes(c->tile_max, c->npiece, false, 4) > 64 * 1024;
+        const bool two12 = 2 * (pp_lds_bytes(c->tile_max, c->npiece, false, 12) + 1536) <= 160 * 1024;
+        c->pp_waves = !big ? 4 : (two12 ? 12 : 16);
     }
     c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
     // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB, for

@@ -580,8 +580,12 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     }
 }
 
+// The FP64 12-wave kernel asks for 6 waves a SIMD (80 VGPR, a 48-byte spill
+// against 95 VGPR): two of its workgroups share a CU where one 16-wave
+// workgroup did (rc 7 at unit density: PP 3.10 -> 2.49 ms).
 template <class R, int NW>
-__global__ __launch_bounds__(NW * 64) void k_pp(PPArgs<R> P) {
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 && NW == 12 ? 6 : 1)))
+void k_pp(PPArgs<R> P) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
     pp_body<R, NW>(P, blk_plain(), ldsraw);
 }
