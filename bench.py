@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--no-liquid", action="store_true",
                     help="measure the force error on the uniform start only (no equilibrated state)")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "drift"), default="n1e6",
+    ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "drift", "sqw"), default="n1e6",
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
     ap.add_argument("--replicas", type=int, default=64)
@@ -450,12 +450,77 @@ def run_drift(a):
     print(json.dumps(res), flush=True)
 
 
+def run_sqw(a):
+    """S(q, omega) post-processing of an ensemble's n(q, t) on the device.
+
+    cfg1's shape (s_of_q_w_script.py:6-25): 20 runs, Nq = int(2.5 / dq) = 13 q
+    (the series n_q_t[:, 3 i, :] of its 39 columns, 3 axes each) of Npd = 16000
+    steps, N = 1e4, dt = 0.05.
+    Synthetic complex n(q, t) already resident in HBM; a step is one complete
+    S(q, omega) of the ensemble (every FFT stage and the |X|^2 pass).  The
+    roofline counts the algorithmic bytes: the 3 Nq used columns read once
+    (16 B a sample) and S written once (8 B per run, q and omega).
+    The CPU baseline is the NumPy oracle (numpy.fft, one thread) on the same
+    ensemble, repeated for at least 5 s.
+    """
+    import numpy as np
+    import torch
+    from mdforce.sqw import sqw
+    dev = torch.device("cuda", 0)
+    n_avg, Nq, npd, N, dt = 20, 13, 16000, 10000, 0.05
+    T = npd * dt
+    g = torch.Generator(device=dev).manual_seed(1)
+    x = torch.randn((n_avg, npd, 3 * Nq, 3), dtype=torch.complex128, device=dev, generator=g)
+    out = torch.empty((n_avg, Nq, npd), dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream(dev)
+    for _ in range(a.warmup):
+        sqw(x, N, dt, T, q_stride=3, nq=Nq, out=out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(a.steps):
+        sqw(x, N, dt, T, q_stride=3, nq=Nq, out=out)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    samples = n_avg * Nq * 3 * npd
+    byts = samples * 16 + n_avg * Nq * npd * 8
+    cpu = None
+    if not a.no_cpu:
+        from oracle import sqw_oracle as O
+        xs = x.cpu().numpy()
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 5.0:
+            O.s_of_q_w_avg_mat(xs, Nq, np.zeros((n_avg, Nq, npd)), n_avg, N, dt, T)
+            reps += 1
+        tc = (time.perf_counter() - t0) / reps
+        cpu = {"value": samples / tc, "unit": "samples/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/sqw_oracle.py s_of_q_w_avg_mat (numpy.fft) on the whole ensemble, "
+                         f"{tc:.3f} s a call, {reps} calls"}
+    res = {
+        "metric": "S(q,w) of an n(q,t) ensemble: time samples transformed per s, 1 GPU",
+        "value": samples / (ms / 1e3), "unit": "samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"S(q,w): {n_avg} runs x {Nq} q x 3 axes x {npd} steps (FP64 mixed-radix FFT "
+                               f"along t, mean over axes)", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": byts / (ms / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": byts / (ms / 1e3) / 1e9 / 8000.0, "traffic": None,
+                     "kernel": "whole S(q,w) call (7 FFT stages + |X|^2)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "sqw":
+        run_sqw(a)
+        return
     if a.workload == "replicas":
         run_replicas(a)
         return

@@ -18,6 +18,8 @@
 #ifndef MDFORCE_H
 #define MDFORCE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -221,6 +223,24 @@ int mdf_mlp_stats(mdf_mlp *mlp, int n, int *out2);
 /* Attach (or detach with NULL) a network to a context: mdf_p3m_force and
  * mdf_md_run then add its correction to the PP force of every evaluation. */
 int mdf_p3m_attach_mlp(mdf_p3m_ctx *ctx, mdf_mlp *mlp);
+
+/* ---- S(q, omega) from the density modes n(q, t) ------------------------
+ * Replaces n_of_q_w (s_of_q_w.py:3-20) and s_of_q_w_avg_mat
+ * (s_of_q_w_avg.py:5-20): per run and q index i < nq, the three axis series
+ * n(t) = nqt[run, t, i * q_stride, axis] are transformed along t (FP64
+ * mixed-radix FFT, any npd >= 1) and S = |dt * FFT(n)|^2 / (N * T), with T the
+ * total time npd * dt of the reference's scripts.
+ *   nqt : device complex128 (nrun, npd, nq_total, 3), the reference's n_q_t
+ *         of each run stacked (mdf_md_run's nqt layout); q_stride 3 takes the
+ *         columns i * 3 as s_of_q_w_avg_mat does, 1 takes every column
+ *   out : device double, (nrun, nq, npd) means over the axes when mean_axes
+ *         != 0 (s_of_q_w_avg_mat's sqw_mat), else (nrun, nq, npd, 3) (each
+ *         series as n_of_q_w returns it)
+ *   work: device scratch of at least mdf_sqw_workspace bytes
+ * Asynchronous on stream; allocates nothing. */
+int mdf_sqw_workspace(int nrun, int npd, int nq, size_t *bytes);
+int mdf_sqw(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N, double dt,
+            double T, int mean_axes, double *out, void *work, size_t work_bytes, void *stream);
 
 #ifdef __cplusplus
 }
