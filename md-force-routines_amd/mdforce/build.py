@@ -60,7 +60,26 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    _build_cli(verbose)
     return LIB
+
+
+CLI = os.path.join(ROOT, "mdf_bench")
+
+
+def _build_cli(verbose: bool) -> None:
+    """The standalone PP/PM timing program, a C caller of libmdforce."""
+    src = os.path.join(CSRC, "cli", "mdf_bench.cpp")
+    inc = os.path.join(os.path.dirname(ROOT), "include")
+    tmp = CLI + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{inc}", src, "-o", tmp, f"-L{PKG_DIR}", "-lmdforce",
+           "-Wl,-rpath,$ORIGIN/mdforce"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"mdf_bench build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CLI)
+    if verbose:
+        print(f"built {CLI}")
 
 
 if __name__ == "__main__":
