@@ -70,6 +70,15 @@ int mdf_p3m_set_green(mdf_p3m_ctx *ctx, const double *G_k, void *stream);
 int mdf_p3m_build_green(mdf_p3m_ctx *ctx, int p, int mx_max, int my_max, int mz_max,
                         double *G_k_out, double *A_pm_out, void *stream);
 
+/* Mesh error functional for the error estimate: Q_sum_out (device double) <-
+ * sum over k != 0 of [sum_m |k_m|^2 G(k_m)^2 - A_k], with G the optimal
+ * influence function of the same p and alias ranges and A_k the k term of
+ * A_pm (yukawa_gf_opt.py:96, whose A_pm the reference returns unused).  For N
+ * random unit charges in volume V the RMS mesh force error is
+ * sqrt(N Q_sum) / V (Hockney-Eastwood).  Writes no Green function. */
+int mdf_p3m_pm_error_sum(mdf_p3m_ctx *ctx, int p, int mx_max, int my_max, int mz_max, double *Q_sum_out,
+                         void *stream);
+
 /* Full P3M force and energy: replaces force_pot (yukawa_p3m_ord6.py:6-29).
  *   acc       <- af * (acc_pp + acc_pm)                       (n,3)
  *   energies  <- [U, U_pp, U_pm, U_self] (device, 4 doubles), U = uf*(U_pp+U_pm+U_self),

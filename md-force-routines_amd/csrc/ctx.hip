@@ -423,6 +423,15 @@ int mdf_p3m_build_green(mdf_p3m_ctx *ctx, int p, int mx_max, int my_max, int mz_
     return build_green(ctx, p, mx_max, my_max, mz_max, G_k_out, A_pm_out, (hipStream_t)stream);
 }
 
+int mdf_p3m_pm_error_sum(mdf_p3m_ctx *ctx, int p, int mx_max, int my_max, int mz_max, double *Q_sum_out,
+                         void *stream) {
+    if (!ctx) {
+        set_error("mdf_p3m_pm_error_sum: null context");
+        return MDF_EINVAL;
+    }
+    return pm_error_sum(ctx, p, mx_max, my_max, mz_max, Q_sum_out, (hipStream_t)stream);
+}
+
 int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, double af, double uf, double *acc,
                   double *energies, double *acc_pp, double *acc_pm, double *e_x, double *e_y, double *e_z,
                   double *rho, void *stream) {

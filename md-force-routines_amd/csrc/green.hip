@@ -27,6 +27,7 @@ struct GreenArgs {
     double *__restrict__ half;
     double *__restrict__ full;
     double *__restrict__ part;
+    int q_mode;  // 1: reduce the error functional terms instead of writing G
 };
 
 __device__ __forceinline__ double ipow(double x, int p) {
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void k_green_build(GreenArgs P) {
         const double k_sq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
         double G = 0.0;
         if (k_sq != 0.0) {
-            double U_k_sq = 0.0, U_G_k = 0.0;
+            double U_k_sq = 0.0, U_G_k = 0.0, R_sq = 0.0;
             for (int mz = 0; mz <= 2 * P.mm[2]; ++mz)
                 for (int my = 0; my <= 2 * P.mm[1]; ++my)
                     for (int mx = 0; mx <= 2 * P.mm[0]; ++mx) {
@@ -72,15 +73,19 @@ __global__ __launch_bounds__(256) void k_green_build(GreenArgs P) {
                         const double kdot = k[0] * kx + k[1] * ky + k[2] * kz;
                         U_G_k += U2 * GM * kdot;
                         U_k_sq += U2;
+                        R_sq += k_M_sq * GM * GM;
                     }
             G = U_G_k / ((U_k_sq * U_k_sq) * k_sq);
             contrib = U_G_k * U_G_k / ((U_k_sq * U_k_sq) * k_sq);
+            if (P.q_mode) contrib = R_sq - contrib;  // this k's term of the error functional
         }
-        if (P.full) P.full[idx] = G;
-        // shifted index n holds natural index (n + M - mid) % M
-        int nat[3];
-        for (int d = 0; d < 3; ++d) nat[d] = (n[d] + P.M[d] - P.M[d] / 2) % P.M[d];
-        if (nat[0] < P.Mxh) P.half[((size_t)nat[2] * P.M[1] + nat[1]) * P.Mxh + nat[0]] = G;
+        if (!P.q_mode) {
+            if (P.full) P.full[idx] = G;
+            // shifted index n holds natural index (n + M - mid) % M
+            int nat[3];
+            for (int d = 0; d < 3; ++d) nat[d] = (n[d] + P.M[d] - P.M[d] / 2) % P.M[d];
+            if (nat[0] < P.Mxh) P.half[((size_t)nat[2] * P.M[1] + nat[1]) * P.Mxh + nat[0]] = G;
+        }
     }
     contrib = wave_sum(contrib);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = contrib;
@@ -130,6 +135,7 @@ int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, d
     P.half = c->green;
     P.full = G_full;
     P.part = c->part_red;
+    P.q_mode = 0;
     const size_t total = c->nmesh;
     const unsigned nb = (unsigned)((total + 255) / 256);
     if (nb > (unsigned)kMaxPartials) {
@@ -144,6 +150,44 @@ int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, d
     }
     c->have_green = true;
     return green_to_float(c, s);
+}
+
+// The Hockney-Eastwood error functional of the optimal influence function,
+// Q_sum = sum_{k != 0} [ sum_m |k_m|^2 G(k_m)^2 - A_k ], A_k the k term of
+// A_pm (the same alias sums as build_green).  For N random unit charges in a
+// box of volume V the mesh force error is sqrt(N Q_sum) / V.
+int pm_error_sum(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *Q_sum, hipStream_t s) {
+    if (mx < 0 || my < 0 || mz < 0 || mx > 3 || my > 3 || mz > 3 || p < 1 || p > 16 || !Q_sum) {
+        set_error("pm_error_sum: need 0 <= m_max <= 3, 1 <= p <= 16 and an output");
+        return MDF_EINVAL;
+    }
+    GreenArgs P;
+    for (int d = 0; d < 3; ++d) {
+        P.M[d] = c->g.M[d];
+        P.h[d] = c->g.h[d];
+        P.L[d] = c->g.L[d];
+    }
+    P.Mxh = c->g.Mxh;
+    P.kappa_sq = c->g.kappa * c->g.kappa;
+    P.Gew_sq = c->g.alpha * c->g.alpha;
+    P.p = p;
+    P.mm[0] = mx;
+    P.mm[1] = my;
+    P.mm[2] = mz;
+    P.half = nullptr;
+    P.full = nullptr;
+    P.part = c->part_red;
+    P.q_mode = 1;
+    const unsigned nb = (unsigned)((c->nmesh + 255) / 256);
+    if (nb > (unsigned)kMaxPartials) {
+        set_error("pm_error_sum: mesh too large");
+        return MDF_EINVAL;
+    }
+    hipLaunchKernelGGL(k_green_build, dim3(nb), dim3(256), 0, s, P);
+    MDF_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, c->part_red, (int)nb, Q_sum);
+    MDF_LAUNCHED();
+    return MDF_OK;
 }
 
 int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s) {

@@ -169,6 +169,7 @@ int finalize_energy(mdf_p3m_ctx *c, int n, double uf, int have_pp, int have_pm, 
                     hipStream_t s);
 int build_green(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *G_full, double *A_pm,
                 hipStream_t s);
+int pm_error_sum(mdf_p3m_ctx *c, int p, int mx, int my, int mz, double *Q_sum, hipStream_t s);
 int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
 
 // PP pair-term tables (host) and the PP kernel's dynamic LDS size.

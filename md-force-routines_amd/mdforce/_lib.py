@@ -79,6 +79,7 @@ SIGNATURES = {
     "mdf_mlp_adam": [_vp, ct.c_float, ct.c_float, ct.c_float, ct.c_float, _vp],
     "mdf_mlp_stats": [_vp, _i, ct.POINTER(ct.c_int)],
     "mdf_p3m_attach_mlp": [_vp, _vp],
+    "mdf_p3m_pm_error_sum": [_vp, _i, _i, _i, _i, _vp, _vp],
     "mdf_sqw_workspace": [_i, _i, _i, ct.POINTER(ct.c_size_t)],
     "mdf_sqw": [_vp, _i, _i, _i, _i, _i, _d, _d, _d, _i, _vp, _vp, ct.c_size_t, _vp],
 }
