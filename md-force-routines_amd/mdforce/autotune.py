@@ -87,8 +87,12 @@ def reference_force(P3M, pos, L, kappa, device=None):
 
 
 def tune(pos, L, kappa, budget=1e-5, meshes=(32, 64), rcs=None, alphas=None, precision="mixed", margin=0.8,
-         ref=None, device=None, fallback_meshes=(128,)):
+         ref=None, device=None, fallback_meshes=(128,), prescreen=4.0):
     """Cheapest (mesh, alpha, rc) with measured error <= margin * budget.
+
+    prescreen: a candidate whose a-priori error estimate (mdforce.error_estimate,
+    random positions) exceeds prescreen * budget of the reference's mean force
+    is recorded with error None and not built or measured; None measures all.
 
     pos: (N, 3) float64 GPU tensor of the system's positions.  The meshes in
     ``fallback_meshes`` are tried only when none of ``meshes`` meets the
@@ -96,14 +100,15 @@ def tune(pos, L, kappa, budget=1e-5, meshes=(32, 64), rcs=None, alphas=None, pre
     candidate meets the budget.
     """
     try:
-        return _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device)
+        return _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device, prescreen)
     except RuntimeError:
         if not fallback_meshes:
             raise
-        return _tune(pos, L, kappa, budget, fallback_meshes, rcs, alphas, precision, margin, ref, device)
+        return _tune(pos, L, kappa, budget, fallback_meshes, rcs, alphas, precision, margin, ref, device, prescreen)
 
 
-def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device):
+def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device, prescreen=None):
+    from . import error_estimate
     from .engine import P3M
 
     N = pos.shape[0]
@@ -118,6 +123,7 @@ def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, de
         alphas = (0.25, 0.3, 0.35, 0.42, 0.5, 0.6, 0.72, 0.86, 1.03)
     cands = sorted(((model_cost(N, M, rc), M, rc) for M in meshes for rc in rcs))
     tried = []
+    mean_f = None
     for cost, M, rc in cands:
         best = None
         for a in alphas:
@@ -128,6 +134,13 @@ def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, de
                 e = P3M(N, M, L, kappa, a, rc, precision=precision, device=device)
             except MdforceError:  # geometry the engine refuses (e.g. too many cells for a small rc)
                 break
+            if prescreen is not None:
+                if mean_f is None:
+                    mean_f = float(torch.mean(torch.sqrt(torch.sum(ref * ref, 1))))
+                if error_estimate.estimate(e, N)["total"] > prescreen * budget * mean_f:
+                    e.close()
+                    tried.append((M, a, rc, None))
+                    continue
             e.build_green()
             acc, _ = e.force(pos, None, af=1.0)
             err = force_error(acc, ref)
