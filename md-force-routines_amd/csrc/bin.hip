@@ -1,8 +1,8 @@
 // Counting-sort binning of particles into PP sub-cells.
 //
-// Replaces the serial head/linked-list build of yukawa_pp.py:72-80 (and
+// Replaces the serial head/linked-list build of yukawa_pp.py:44-52 (and
 // yukawa_p3m.c:265-291).  Sub-cells are half a reference cell wide (the
-// reference cell is L/floor(L/rc), yukawa_pp.py:45-51), so a 2x2x2 block of
+// reference cell is L/floor(L/rc), yukawa_pp.py:17-23), so a 2x2x2 block of
 // sub-cells is exactly one reference cell.  Three kernels:
 //   1. count: sub-cell of each particle, slot = atomicAdd(count[cell])
 //   2. scan : exclusive prefix sum of the counts (and reset of the counts)
@@ -12,7 +12,7 @@
 // the periodic image at 0); in-range positions are copied bit for bit.
 // Every kernel that turns a position into a cell or mesh index reads the
 // sorted copy, so the wrap must hold for any input: the Verlet drift wraps
-// once per step (velocity_verlet_p3m_ord6.py:15-22), and an ion faster than
+// once per step (velocity_verlet_p3m_ord6.py:12-19), and an ion faster than
 // L/dt leaves the box by more than one period.  Such positions are reduced
 // with floor(x / L); a non-finite one is placed at 0.
 #include "mdf_internal.h"

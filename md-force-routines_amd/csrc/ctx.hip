@@ -152,6 +152,21 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             return MDF_EINVAL;
         }
     }
+    {
+        // cell counts (reference cells floor(L/rc) per axis, two sub-cells each),
+        // refused before anything is allocated
+        double nb = 1.0;
+        for (int d = 0; d < 3; ++d) nb *= floor(p->box[d] / p->rc);
+        const double ns = 8.0 * nb;
+        if (ns > 1024.0 * 8192.0) {
+            set_error("too many cells (%.0f sub-cells > %d): increase rc", ns, 1024 * 8192);
+            return MDF_EINVAL;
+        }
+        if (nb > (double)kMaxPartials) {
+            set_error("too many cells (%.0f > %d): increase rc", nb, kMaxPartials);
+            return MDF_EINVAL;
+        }
+    }
     mdf_p3m_ctx *c = new (std::nothrow) mdf_p3m_ctx();
     if (!c) {
         set_error("out of host memory");
@@ -165,7 +180,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         g.M[d] = p->mesh[d];
         g.h[d] = p->box[d] / p->mesh[d];
         g.ih[d] = 1.0 / g.h[d];
-        const int lxd = (int)floor(p->box[d] / p->rc);  // reference cells, yukawa_pp.py:45-47
+        const int lxd = (int)floor(p->box[d] / p->rc);  // reference cells, yukawa_pp.py:17-19
         g.nsub[d] = 2 * lxd;
         g.nblk[d] = lxd;
         g.w[d] = p->box[d] / g.nsub[d];
@@ -176,16 +191,6 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     g.rc = p->rc;
     c->ncell = g.nsub[0] * g.nsub[1] * g.nsub[2];
     c->nblock = g.nblk[0] * g.nblk[1] * g.nblk[2];
-    if (c->ncell > 1024 * 8192) {
-        delete c;
-        set_error("too many cells (%d): increase rc", c->ncell);
-        return MDF_EINVAL;
-    }
-    if (c->nblock > kMaxPartials) {
-        delete c;
-        set_error("too many cells (%d > %d): increase rc", c->nblock, kMaxPartials);
-        return MDF_EINVAL;
-    }
     const bool mixed = p->precision == MDF_PREC_MIXED;
     const size_t es = mixed ? 4 : 8;  // bytes of the pair/mesh arithmetic type
     c->nmesh = (size_t)g.M[0] * g.M[1] * g.M[2];

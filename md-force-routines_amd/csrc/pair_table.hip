@@ -1,7 +1,7 @@
 // Host-side construction of the PP pair-term tables (see pp.hip).
 //
 // g1(r) = r phi(r) and g2(r) = r^2 f(r) of the reference pair terms
-// (yukawa_pp.py:143-152) are fitted on [0, rc] by piecewise Chebyshev
+// (yukawa_pp.py:115-124) are fitted on [0, rc] by piecewise Chebyshev
 // series: npiece equal pieces of a given degree, coefficients from the
 // discrete cosine transform of long-double values at the Chebyshev nodes of
 // each piece.  FP64 tables keep the Chebyshev coefficients (Clenshaw in the

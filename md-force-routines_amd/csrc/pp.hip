@@ -1,9 +1,9 @@
 // Short-range Yukawa-Ewald PP forces and energy on a sub-cell list.
 //
-// Reference: particle_particle, yukawa_pp.py:6-167 (C twin yukawa_p3m.c:182-416).
+// Reference: particle_particle, yukawa_pp.py:6-139 (C twin yukawa_p3m.c:182-416).
 // Same pair set (every pair with minimum-image distance r < rc, rc <= L/3),
-// same pair terms (yukawa_pp.py:143-152), same displacement convention
-// d = x_i - (x_j + shift) (:133-135), same cutoff r < rc (:137-140, tested as
+// same pair terms (yukawa_pp.py:115-124), same displacement convention
+// d = x_i - (x_j + shift) (:105-107), same cutoff r < rc (:109-112, tested as
 // d.d < rc^2; the two differ only for r within an ulp of rc).
 // Only the organisation of the work and the summation order differ:
 //
@@ -147,7 +147,7 @@ __device__ __forceinline__ void pp_pair_cheb(const PPArgs<T> &P, const typename 
                                              T dy, T dz, T &fx, T &fy, T &fz, T &u) {
     constexpr int D = ChebDeg<T>::v;
     const T r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 < P.rc2)) return;  // r < rc (yukawa_pp.py:140) tested as r^2 < rc^2
+    if (!(r2 < P.rc2)) return;  // r < rc (yukawa_pp.py:112) tested as r^2 < rc^2
     if (sizeof(T) == 4 && !(r2 > (T)0)) return;
     const T ir = rsqrt_t<T>(r2);
     const T r = r2 * ir;

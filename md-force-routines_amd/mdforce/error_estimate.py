@@ -3,7 +3,7 @@
 The two parts of the error add in quadrature, each as
 <|dF|^2> = (N / V) * (integral of the squared pair-force error over space):
 * PP, the pairs dropped beyond rc: dF_pp^2 = (N / V) 4 pi int_rc^inf f(r)^2 r^2 dr
-  with f = g2(r) / r^2 the short-range force magnitude (yukawa_pp.py:115-117;
+  with f = g2(r) / r^2 the short-range force magnitude (yukawa_pp.py:121-124;
   g2 as the PP kernel tabulates it), integrated numerically;
 * PM, the mesh force: the Hockney-Eastwood error functional of the optimal
   influence function, dF_pm^2 = N Q_sum / V^2, with

@@ -17,7 +17,10 @@ def _velocities(rng, N, T_desired):
     vel[:, 0] = sig * np.sqrt(-2 * np.log(u1)) * np.cos(2 * np.pi * u2)
     vel[:, 1] = sig * np.sqrt(-2 * np.log(u1)) * np.sin(2 * np.pi * u2)
     vel[:, 2] = sig * np.sqrt(-2 * np.log(u3)) * np.cos(2 * np.pi * u4)
-    vel -= vel.mean(axis=0)
+    # per-column means of the strided columns (:21-28), not vel.mean(axis=0),
+    # whose row-wise accumulation rounds differently from NumPy's pairwise sum
+    for a in range(3):
+        vel[:, a] = vel[:, a] - np.mean(vel[:, a])
     return vel
 
 

@@ -1,4 +1,4 @@
-"""Drop-in for the reference's yukawa_pp (yukawa_pp.py:6-167).
+"""Drop-in for the reference's yukawa_pp (yukawa_pp.py:6-139).
 
 ``particle_particle(kappa, G, rc, Lv, pos, acc_s_r)`` returns
 (U_s_r, acc_s_r) with acc_s_r filled in place (unit charges, as the

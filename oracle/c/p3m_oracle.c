@@ -4,7 +4,7 @@
  * baseline ("port") that bench.py times.  The product path never links it.
  *
  * Follows, function by function:
- *   orc_pp              yukawa_pp.py:6-167 (twin: yukawa_p3m.c:182-416)
+ *   orc_pp              yukawa_pp.py:6-139 (twin: yukawa_p3m.c:182-416)
  *   orc_charge_assign   yukawa_pm_ord6.py:5-95
  *   orc_field_gather    yukawa_pm_ord6.py:212-312
  *   orc_pm              yukawa_pm_ord6.py:439-484 (FFTW -> own radix-2 FFT)
@@ -41,7 +41,7 @@ static double csum_get(const csum *a) { return a->s + a->c; }
 
 /* ---------------------------------------------------------------- PP ---- */
 
-/* Pair energy and radial force exactly as written in yukawa_pp.py:143-152. */
+/* Pair energy and radial force exactly as written in yukawa_pp.py:115-124. */
 static void pair_terms(double r, double kappa, double G, double *phi, double *fr)
 {
     double a = 0.5 * kappa / G;
@@ -70,7 +70,7 @@ int orc_pp(const double *pos, int n, const double *Lv, double kappa, double G, d
     if (!head || !ls) { free(head); free(ls); return -2; }
     for (int c = 0; c < ncell; ++c) head[c] = empty;
     memset(acc, 0, (size_t)n * 3 * sizeof(double));
-    /* head / linked list, yukawa_pp.py:72-80; out-of-range cells clamped */
+    /* head / linked list, yukawa_pp.py:44-52; out-of-range cells clamped */
     for (int i = 0; i < n; ++i) {
         int cc[3];
         for (int d = 0; d < 3; ++d) {

@@ -52,7 +52,11 @@ def maxwell_velocities(rng, N, T_desired):
     vel[:, 0] = sig * np.sqrt(-2 * np.log(u1)) * np.cos(2 * np.pi * u2)
     vel[:, 1] = sig * np.sqrt(-2 * np.log(u1)) * np.sin(2 * np.pi * u2)
     vel[:, 2] = sig * np.sqrt(-2 * np.log(u3)) * np.cos(2 * np.pi * u4)
-    vel -= vel.mean(axis=0)
+    # the mean of each strided column, as :21-28 take it (np.mean of vel[:, a]:
+    # pairwise summation; vel.mean(axis=0) accumulates row by row and rounds
+    # differently)
+    for a in range(3):
+        vel[:, a] = vel[:, a] - np.mean(vel[:, a])
     return vel
 
 
@@ -254,7 +258,7 @@ def particle_mesh_fft_r(kappa, G_ew, pos, Z, N, G_k, kx_v, ky_v, kz_v, Mx, My, M
 
 
 def pp_pair(r, kappa, G):
-    """Pair energy and radial force of yukawa_pp.py:143-152, as written."""
+    """Pair energy and radial force of yukawa_pp.py:115-124, as written."""
     a = 0.5 * kappa / G
     phi = (0.5 / r) * (np.exp(kappa * r) * erfc(G * r + a) + np.exp(-kappa * r) * erfc(G * r - a))
     f1 = (0.5 / r ** 2) * np.exp(kappa * r) * erfc(G * r + a) * (1 - kappa * r)
@@ -265,11 +269,11 @@ def pp_pair(r, kappa, G):
 
 
 def particle_particle(kappa, G, rc, Lv, pos, chunk_pairs=4_000_000, Z=None):
-    """Linked-cell PP sum with i < j (Newton's third law), yukawa_pp.py:6-167.
+    """Linked-cell PP sum with i < j (Newton's third law), yukawa_pp.py:6-139.
 
-    Cells: Lxd = floor(L/rc), width L/Lxd (:45-51); c = cx + cy*Lxd +
-    cz*Lxd*Lyd (:74-77); the 27 neighbour cells with the periodic image shift
-    of :88-122; pair test i < j and r < rc (:132-140).  Pairs are gathered
+    Cells: Lxd = floor(L/rc), width L/Lxd (:17-23); c = cx + cy*Lxd +
+    cz*Lxd*Lyd (:46-49); the 27 neighbour cells with the periodic image shift
+    of :60-94; pair test i < j and r < rc (:104-112).  Pairs are gathered
     per neighbour offset over all cells at once instead of walking the
     linked list; the pair set is identical.  A cell index of Lxd (a position
     exactly at L) is clamped into range, which the reference leaves

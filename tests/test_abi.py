@@ -69,3 +69,21 @@ def test_create_rejects_bad_geometry_without_gpu_work():
     assert b"L/rc >= 3" in lib().mdf_last_error()
     p.rc, p.order = 3.0, 4
     assert lib().mdf_p3m_create(ct.byref(p), ct.byref(ctx)) == -1
+
+
+def test_create_rejects_too_many_cells_before_allocating():
+    # a large box at a small cutoff: 8 floor(L/rc)^3 sub-cells exceed the cell
+    # table, refused before any allocation or device call
+    from mdforce._lib import P3MParams, lib
+    p = P3MParams()
+    p.n_max = 1000
+    for d in range(3):
+        p.mesh[d] = 64
+        p.box[d] = 330.0
+    p.kappa, p.alpha, p.rc, p.order, p.precision = 0.5, 0.8, 3.0, 6, 1
+    ctx = ct.c_void_p()
+    assert lib().mdf_p3m_create(ct.byref(p), ct.byref(ctx)) == -1
+    msg = lib().mdf_last_error()
+    assert b"too many cells" in msg and b"increase rc" in msg
+    assert b"sub-cells" in msg
+    assert not ctx.value

@@ -360,3 +360,134 @@ def test_vscale_branches():
     _, vv, _, _ = O.update(pos, vel, acc, 0.1, P)
     T = np.sum(vv ** 2) / N
     np.testing.assert_allclose(v2, vv * np.sqrt((20 * 0.02 / T - 1) / 20), rtol=1e-14)
+
+
+class _StubForce:
+    """A stand-in for P3MParams whose force is a fixed field (no P3M)."""
+
+    def __init__(self, L, a):
+        self.Lv = np.array([L, L, L])
+        self.a = a
+
+    def force(self, pos, Z=None):
+        return 0.0, np.broadcast_to(self.a, pos.shape).copy(), None
+
+
+def test_update_free_flight_zero_force():
+    """Pins update (velocity_verlet_p3m_ord6.py:7-24) on a zero force field:
+    the step is then free flight, x1 = wrap(x0 + v0 dt) and v1 = v0 exactly,
+    with the single strict wrap of :12-19 (x > L -> x - L, x < 0 -> x + L)."""
+    rng = np.random.RandomState(7)
+    N, L, dt = 300, 10.0, 0.1
+    pos = rng.random_sample((N, 3)) * L
+    vel = (rng.random_sample((N, 3)) - 0.5) * 30.0  # up to 1.5 L per step: some leave the box
+    P = _StubForce(L, np.zeros(3))
+    p1, v1, a1, _ = O.update(pos, vel, np.zeros((N, 3)), dt, P)
+    assert np.array_equal(v1, vel)
+    free = pos + vel * dt
+    want = free.copy()
+    want[free > L] -= L
+    want[want < 0] += L
+    assert np.array_equal(p1, want)
+    assert ((free > L) | (free < 0)).any()  # the wrap branch was exercised
+    assert not np.any(a1)
+
+
+def test_update_constant_force_closed_form():
+    """Velocity Verlet integrates a constant acceleration a exactly:
+    x(t) = x0 + v0 t + a t^2 / 2, v(t) = v0 + a t, to rounding.  A drift before
+    the first half kick, or a full kick in place of a half kick, gives
+    x0 + v0 t + a t^2 or x0 + v0 t, off by a t^2 / 2 (here ~1e-2 of L)."""
+    rng = np.random.RandomState(11)
+    N, L, dt, nstep = 100, 1.0e6, 0.1, 20  # box large enough that nothing wraps
+    pos = rng.random_sample((N, 3)) * 10.0 + L / 2
+    vel = rng.standard_normal((N, 3))
+    a = np.array([0.3, -0.7, 1.1])
+    P = _StubForce(L, a)
+    p, v, acc = pos, vel, np.broadcast_to(a, pos.shape).copy()
+    for _ in range(nstep):
+        p, v, acc, _ = O.update(p, v, acc, dt, P)
+    t = nstep * dt
+    np.testing.assert_allclose(v, vel + a * t, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(p - L / 2, pos - L / 2 + vel * t + 0.5 * a * t * t, rtol=0, atol=1e-9)
+
+
+def test_update_time_reversal_with_p3m_forces():
+    """Velocity Verlet is time reversible: 20 steps forward, negate the
+    velocities, 20 steps back return every ion to its start (N = 400 with the
+    oracle's own P3M forces).  A mis-ordered kick and drift breaks this."""
+    N = 400
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 2.0, 0.9, L / 3.1, 16)
+    # a warm start (T = 1) so that ions move a good fraction of a spacing
+    pos, vel = O.initial_split(5, 6, N, P.Lv, 1.0)
+    _, acc, _ = P.force(pos)
+    p, v, a = pos.copy(), vel.copy(), acc.copy()
+    for _ in range(20):
+        p, v, a, _ = O.update(p, v, a, 0.05, P)
+    moved = np.abs(p - pos)
+    moved = np.minimum(moved, L - moved)
+    assert moved.max() > 0.3  # the trajectory went somewhere
+    v = -v
+    for _ in range(20):
+        p, v, a, _ = O.update(p, v, a, 0.05, P)
+    d = np.abs(p - pos)
+    d = np.minimum(d, L - d)
+    assert d.max() < 1e-10 * L
+    np.testing.assert_allclose(-v, vel, rtol=0, atol=1e-10)
+
+
+def test_update_energy_error_is_second_order():
+    """The energy error of velocity Verlet over a fixed time falls as dt^2:
+    halving dt quarters it.  An update with the force sign flipped, or a
+    first-order splitting, fails this (no conservation, or a factor 2)."""
+    N = 200
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 2.0, 0.9, L / 3.1, 16)
+    pos0, vel0 = O.initial_split(8, 9, N, P.Lv, 0.5)
+    U0, acc0, _ = P.force(pos0)
+    E0 = U0 + P.kf * np.sum(vel0 ** 2)
+    errs = []
+    for dt, nstep in ((0.04, 10), (0.02, 20), (0.01, 40)):
+        p, v, a = pos0.copy(), vel0.copy(), acc0.copy()
+        for _ in range(nstep):
+            p, v, a, U = O.update(p, v, a, dt, P)
+        errs.append(abs(U + P.kf * np.sum(v ** 2) - E0))
+    assert errs[0] > 1e-9 * abs(E0)  # well above rounding
+    for e1, e2 in zip(errs, errs[1:]):
+        assert 3.0 < e1 / e2 < 5.5
+
+
+def test_dropin_initial_matches_oracle_and_global_stream():
+    """The drop-in initial (initialize_pos_vel.py:4-35 signature: fills pos
+    and vel in place) draws the same numbers as the oracle, and both equal
+    the reference's own sequence of global-state calls: np.random.seed, four
+    np.random.random(N) for Box-Muller, the per-column mean removal of
+    :21-28, then three position columns (:31-33)."""
+    from mdforce.initialize_pos_vel import initial as dropin_initial
+
+    N, T = 1000, 0.01
+    Lv = np.array([30.0, 31.0, 32.5])
+    pos = np.empty((N, 3))
+    vel = np.empty((N, 3))
+    p_ret, v_ret = dropin_initial(17, N, Lv, pos, vel, T)
+    assert p_ret is pos and v_ret is vel
+    po, vo = O.initial(17, N, Lv, T)
+    assert np.array_equal(pos, po) and np.array_equal(vel, vo)
+    state = np.random.get_state()
+    try:
+        np.random.seed(seed=17)
+        sig = np.sqrt(T / 3)
+        u1, u2, u3, u4 = (np.random.random(N) for _ in range(4))
+        want_v = np.empty((N, 3))
+        want_v[:, 0] = sig * np.sqrt(-2 * np.log(u1)) * np.cos(2 * np.pi * u2)
+        want_v[:, 1] = sig * np.sqrt(-2 * np.log(u1)) * np.sin(2 * np.pi * u2)
+        want_v[:, 2] = sig * np.sqrt(-2 * np.log(u3)) * np.cos(2 * np.pi * u4)
+        means = [np.mean(want_v[:, a]) for a in range(3)]
+        for a in range(3):
+            want_v[:, a] = want_v[:, a] - means[a]
+        want_p = np.stack([Lv[a] * np.random.random(N) for a in range(3)], 1)
+    finally:
+        np.random.set_state(state)
+    assert np.array_equal(vel, want_v)
+    assert np.array_equal(pos, want_p)
