@@ -25,6 +25,7 @@
 // Steps beyond the last whole graph run eagerly on the caller stream.
 #include <algorithm>
 #include <cstring>
+#include <cstdio>
 #include <vector>
 
 #include "mdf_internal.h"
@@ -509,6 +510,16 @@ int mdf_batch_run(mdf_batch *b, double *const *pos, double *const *vel, double *
             MDF_TRY(md_step(c, pos[r], vel[r], acc[r], nullptr, n[r], dt, af, uf, kf, rec, s));
             if (nq > 0)
                 MDF_TRY(density_modes_grid(c, pos[r], n[r], q_step_of(c), nq, nqt[r] + (size_t)it * nq * 6, s));
+        }
+    }
+    // the graph's work ran on the replicas' own streams, joined back into s
+    for (int r = 0; r < b->nrep; ++r) {
+        const int rc = check_overflow(b->ctx[r], b->ctx[r]->mlp, s);
+        if (rc != MDF_OK) {
+            char msg[1024];
+            snprintf(msg, sizeof msg, "replica %d: %s", r, mdf_last_error());
+            set_error("%s", msg);
+            return rc;
         }
     }
     return MDF_OK;

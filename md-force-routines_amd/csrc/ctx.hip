@@ -106,6 +106,27 @@ static int need_green(mdf_p3m_ctx *c) {
     return MDF_OK;
 }
 
+int check_overflow(mdf_p3m_ctx *c, mdf_mlp *m, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MDF_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return MDF_OK;
+    int h[4] = {0, 0, 0, 0}, hm = 0;
+    MDF_HIP(hipMemcpyAsync(h, c->stats, sizeof h, hipMemcpyDeviceToHost, s));
+    if (m) MDF_HIP(hipMemcpyAsync(&hm, mlp_flag(m), sizeof hm, hipMemcpyDeviceToHost, s));
+    MDF_HIP(hipStreamSynchronize(s));
+    if (!h[2] && !hm) return MDF_OK;
+    if (h[2]) MDF_TRY(fill_zero_i(c->stats + 2, 1, s));
+    if (hm) MDF_TRY(fill_zero_i(mlp_flag(m), 1, s));
+    MDF_HIP(hipStreamSynchronize(s));
+    if (h[2])
+        set_error("a PP tile row held more particles than the LDS tile capacity (%d): its pairs were not "
+                  "evaluated; the density is beyond what this context was sized for (raise n_max)",
+                  c->tile_max);
+    else
+        set_error("the learned correction's pair list overflowed its capacity: its pairs were not all evaluated");
+    return MDF_ESTATE;
+}
+
 }  // namespace mdf
 
 using namespace mdf;
@@ -470,7 +491,7 @@ int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, d
     prof_mark(ctx, kStageFinal, s);
     MDF_TRY(finalize_energy(ctx, n, uf, 1, 1, energies, s));
     prof_mark(ctx, kStageEnd, s);
-    return MDF_OK;
+    return check_overflow(ctx, ctx->mlp, s);
 }
 
 int mdf_pp_force(mdf_p3m_ctx *ctx, const double *pos, int n, double *acc_pp, double *u_pp, void *stream) {
@@ -485,7 +506,7 @@ int mdf_pp_force(mdf_p3m_ctx *ctx, const double *pos, int n, double *acc_pp, dou
     if (acc_pp) MDF_TRY(unpermute3(ctx, ctx->acc_pp_s, acc_pp, n, s));
     MDF_TRY(finalize_energy(ctx, n, 1.0, 1, 0, nullptr, s));
     if (u_pp) MDF_HIP(hipMemcpyAsync(u_pp, ctx->scal + kScalUpp, sizeof(double), hipMemcpyDeviceToDevice, s));
-    return MDF_OK;
+    return check_overflow(ctx, nullptr, s);
 }
 
 int mdf_pm_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, double *acc_pm, double *u_pm,
@@ -573,7 +594,7 @@ int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const do
         }
         prof_mark(ctx, kStageEnd, s);
     }
-    return MDF_OK;
+    return check_overflow(ctx, ctx->mlp, s);
 }
 
 int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout) {

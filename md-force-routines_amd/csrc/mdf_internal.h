@@ -180,6 +180,14 @@ size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw);
 int green_to_float(mdf_p3m_ctx *c, hipStream_t s);
 // Learned correction of the attached network, added to acc_pp_s.
 int mlp_correct(mdf_mlp *m, mdf_p3m_ctx *c, int n, hipStream_t s);
+// The network's pair-list overflow flag (device int).
+int *mlp_flag(mdf_mlp *m);
+// Overflow flags that the kernels set where they would otherwise leave work
+// undone: a PP tile row over the LDS capacity (ctx stats[2]) and, when m is
+// not null, a pair list over the network's capacity.  Waits for the stream's
+// work, returns MDF_ESTATE with a message if either is set and clears both.
+// A no-op on a stream that is being captured into a graph.
+int check_overflow(mdf_p3m_ctx *c, mdf_mlp *m, hipStream_t s);
 
 // Fill helpers (kernels, so they can live inside captured graphs).
 int fill_zero(double *p, size_t n, hipStream_t s);

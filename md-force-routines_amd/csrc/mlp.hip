@@ -793,6 +793,8 @@ int mlp_correct(mdf_mlp *m, mdf_p3m_ctx *c, int n, hipStream_t s) {
     return MDF_OK;
 }
 
+int *mlp_flag(mdf_mlp *m) { return m->flag; }
+
 // Loss and gradient for targets (n,3) in caller order; the pair list must be built.
 static int mlp_grad(mdf_mlp *m, mdf_p3m_ctx *c, const double *target, int n, hipStream_t s) {
     MDF_TRY(mlp_forward(m, n, s));
@@ -917,6 +919,7 @@ int mdf_mlp_pairs(mdf_mlp *m, mdf_p3m_ctx *ctx, const double *pos, int n, int *p
     MDF_HIP(hipMemcpyAsync(&flag, m->flag, sizeof(int), hipMemcpyDeviceToHost, s));
     MDF_HIP(hipStreamSynchronize(s));
     if (flag) {
+        MDF_TRY(fill_zero_i(m->flag, 1, s));
         set_error("mdf_mlp_pairs: %d pairs exceed max_pairs = %d", total, m->max_pairs);
         return MDF_ESTATE;
     }
@@ -944,7 +947,8 @@ int mdf_mlp_correction(mdf_mlp *m, mdf_p3m_ctx *ctx, const double *pos, int n, d
     MDF_TRY(mlp_forward(m, n, s));
     hipLaunchKernelGGL(k_mlp_force<1>, dim3((n + 255) / 256), dim3(256), 0, s, m->pairs, m->cnt, m->g, n, m->max_pairs, m->fpred);
     MDF_LAUNCHED();
-    return unpermute3(ctx, m->fpred, f_out, n, s);
+    MDF_TRY(unpermute3(ctx, m->fpred, f_out, n, s));
+    return check_overflow(ctx, m, s);
 }
 
 int mdf_mlp_grad(mdf_mlp *m, mdf_p3m_ctx *ctx, const double *pos, const double *target, int n, float *grad_out,
@@ -958,6 +962,7 @@ int mdf_mlp_grad(mdf_mlp *m, mdf_p3m_ctx *ctx, const double *pos, const double *
     if (build) {
         MDF_TRY(bin_particles(ctx, pos, nullptr, n, s));
         MDF_TRY(mlp_build_pairs(m, ctx, n, s));
+        MDF_TRY(check_overflow(ctx, m, s));
     }
     MDF_TRY(mlp_grad(m, ctx, target, n, s));
     if (grad_out) MDF_HIP(hipMemcpyAsync(grad_out, m->grad, kNParam * sizeof(float), hipMemcpyDeviceToDevice, s));

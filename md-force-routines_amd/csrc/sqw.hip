@@ -182,9 +182,36 @@ extern "C" int mdf_sqw_workspace(int nrun, int npd, int nq, size_t *bytes) {
     return MDF_OK;
 }
 
+namespace mdf {
+namespace {
+
+// Every stage launches one thread per work item, at most nseries * npd of
+// them, from a 32-bit block count: capped well inside the 2^32 threads of a
+// HIP grid.
+constexpr long long kSqwMaxItems = (1ll << 31) - 256;
+
+int sqw_run(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N, double dt, double T,
+            int mean_axes, double *out, void *work, size_t work_bytes, hipStream_t s);
+
+}  // namespace
+}  // namespace mdf
+
 extern "C" int mdf_sqw(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N,
                        double dt, double T, int mean_axes, double *out, void *work, size_t work_bytes,
                        void *stream) {
+    if (!work) {
+        set_error("mdf_sqw: null workspace");
+        return MDF_EINVAL;
+    }
+    return sqw_run(nqt, nrun, npd, nq_total, q_stride, nq, N, dt, T, mean_axes, out, work, work_bytes,
+                   (hipStream_t)stream);
+}
+
+namespace mdf {
+namespace {
+
+int sqw_run(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N, double dt, double T,
+            int mean_axes, double *out, void *work, size_t work_bytes, hipStream_t s) {
     if (!nqt || !out || !work || nrun < 1 || npd < 1 || nq < 1 || q_stride < 1 ||
         (long long)(nq - 1) * q_stride >= nq_total) {
         set_error("mdf_sqw: bad arguments (nrun %d, npd %d, nq_total %d, q_stride %d, nq %d)", nrun, npd, nq_total,
@@ -202,11 +229,10 @@ extern "C" int mdf_sqw(const double *nqt, int nrun, int npd, int nq_total, int q
         return MDF_EINVAL;
     }
     const long long nseries = (long long)nrun * nq * 3;
-    if (nseries * npd > (1ll << 40) || nseries > (1ll << 30)) {
-        set_error("mdf_sqw: too many series elements");
+    if (nseries * npd > kSqwMaxItems) {
+        set_error("mdf_sqw: %lld series samples exceed the %lld a call can take", nseries * npd, kSqwMaxItems);
         return MDF_EINVAL;
     }
-    hipStream_t s = (hipStream_t)stream;
     double2 *buf[2] = {(double2 *)work, (double2 *)work + nseries * npd};
     SqwIn in;
     in.p = (const double2 *)nqt;
@@ -249,3 +275,6 @@ extern "C" int mdf_sqw(const double *nqt, int nrun, int npd, int nq_total, int q
     MDF_LAUNCHED();
     return MDF_OK;
 }
+
+}  // namespace
+}  // namespace mdf

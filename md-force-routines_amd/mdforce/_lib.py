@@ -17,7 +17,14 @@ PREC_MIXED = 1
 
 
 class MdforceError(RuntimeError):
-    """An mdf_* call returned an error code."""
+    """An mdf_* call returned an error code (``code``: the MDF_E* value)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+MDF_EINVAL, MDF_ENOMEM, MDF_EHIP, MDF_ESTATE = -1, -2, -3, -4
 
 
 class P3MParams(ct.Structure):
@@ -117,4 +124,4 @@ def last_error() -> str:
 
 def check(rc: int, what: str) -> None:
     if rc != 0:
-        raise MdforceError(f"{what} failed (code {rc}): {last_error()}")
+        raise MdforceError(f"{what} failed (code {rc}): {last_error()}", rc)

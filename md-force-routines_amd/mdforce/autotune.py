@@ -27,7 +27,12 @@ from dataclasses import dataclass, field
 
 import torch
 
-from ._lib import MdforceError
+from ._lib import MDF_EINVAL, MdforceError
+
+
+class NoCandidateError(RuntimeError):
+    """No (mesh, alpha, rc) of the search meets the force budget."""
+
 
 A_PP = 4.7e-12
 A_PM = 3.7e-10
@@ -96,12 +101,13 @@ def tune(pos, L, kappa, budget=1e-5, meshes=(32, 64), rcs=None, alphas=None, pre
 
     pos: (N, 3) float64 GPU tensor of the system's positions.  The meshes in
     ``fallback_meshes`` are tried only when none of ``meshes`` meets the
-    budget (weak screening in a large box).  Returns a Choice; raises if no
-    candidate meets the budget.
+    budget (weak screening in a large box).  Returns a Choice; raises
+    NoCandidateError if no candidate meets the budget.  Any other failure
+    (a device allocation, a failed force call) propagates as raised.
     """
     try:
         return _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, device, prescreen)
-    except RuntimeError:
+    except NoCandidateError:
         if not fallback_meshes:
             raise
         return _tune(pos, L, kappa, budget, fallback_meshes, rcs, alphas, precision, margin, ref, device, prescreen)
@@ -132,7 +138,9 @@ def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, de
                 continue
             try:
                 e = P3M(N, M, L, kappa, a, rc, precision=precision, device=device)
-            except MdforceError:  # geometry the engine refuses (e.g. too many cells for a small rc)
+            except MdforceError as err:
+                if err.code != MDF_EINVAL:  # only a refused geometry (too many cells for a small rc) is skipped
+                    raise
                 break
             if prescreen is not None:
                 if mean_f is None:
@@ -150,4 +158,4 @@ def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, de
                 best = (a, err)
         if best is not None and best[1] <= margin * budget:
             return Choice(M, best[0], rc, best[1], cost, tried)
-    raise RuntimeError(f"no (mesh, alpha, rc) meets the force budget {budget} (N={N}, L={L:.3f}, kappa={kappa})")
+    raise NoCandidateError(f"no (mesh, alpha, rc) meets the force budget {budget} (N={N}, L={L:.3f}, kappa={kappa})")

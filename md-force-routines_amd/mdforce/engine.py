@@ -23,11 +23,11 @@ def _ptr(t):
     return ct.c_void_p(t.data_ptr())
 
 
-def _need(t, name, shape=None, device=None):
+def _need(t, name, shape=None, device=None, dtype=torch.float64):
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name} must be a torch tensor")
-    if t.dtype != torch.float64:
-        raise TypeError(f"{name} must be float64, got {t.dtype}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
     if not t.is_cuda:
         raise ValueError(f"{name} must be a GPU tensor")
     if device is not None and t.device != device:
@@ -194,6 +194,10 @@ class P3M:
             _need(Z, "Z", (n,), self.device)
         if acc_pm is None:
             acc_pm = torch.empty((n, 3), dtype=torch.float64, device=self.device)
+        for t, name, shp in ((acc_pm, "acc_pm", (n, 3)), (e_x, "e_x", (n,)), (e_y, "e_y", (n,)), (e_z, "e_z", (n,)),
+                             (rho, "rho", (self.mesh[2], self.mesh[1], self.mesh[0]))):
+            if t is not None:
+                _need(t, name, shp, self.device)
         u = self._scratch[1:2]
         check(self._lib.mdf_pm_force(self._ctx, _ptr(pos), _ptr(Z), n, _ptr(acc_pm), _ptr(u), _ptr(e_x),
                                      _ptr(e_y), _ptr(e_z), _ptr(rho), self._stream()), "mdf_pm_force")
@@ -202,6 +206,8 @@ class P3M:
     # ---------------------------------------------------------- integrator --
     def kick_drift(self, pos, vel, acc, dt, pbc=1, lmin=None, lmax=None):
         n = pos.shape[0]
+        for t, name in ((pos, "pos"), (vel, "vel"), (acc, "acc")):
+            _need(t, name, (n, 3), self.device)
         lo = (ct.c_double * 3)(*(lmin if lmin is not None else (0.0, 0.0, 0.0)))
         hi = (ct.c_double * 3)(*(lmax if lmax is not None else self.box))
         check(self._lib.mdf_verlet_kick_drift(self._ctx, _ptr(pos), _ptr(vel), _ptr(acc), n, float(dt), int(pbc),
@@ -210,21 +216,35 @@ class P3M:
 
     def kick(self, vel, acc, dt, k_sum=None):
         n = vel.shape[0]
+        _need(vel, "vel", (n, 3), self.device)
+        _need(acc, "acc", (n, 3), self.device)
+        if k_sum is not None:
+            _need(k_sum, "k_sum", None, self.device)
+            if k_sum.numel() < 1:
+                raise ValueError("k_sum must hold one double")
         check(self._lib.mdf_verlet_kick(self._ctx, _ptr(vel), _ptr(acc), n, float(dt), _ptr(k_sum),
                                         self._stream()), "mdf_verlet_kick")
 
     def thermostat(self, vel, k_sum, T_desired, it, t_out=None):
         n = vel.shape[0]
+        _need(vel, "vel", (n, 3), self.device)
+        for t, name in ((k_sum, "k_sum"), (t_out, "t_out")):
+            if t is not None:
+                _need(t, name, None, self.device)
+                if t.numel() < 1:
+                    raise ValueError(f"{name} must hold one double")
         check(self._lib.mdf_thermostat_scale(self._ctx, _ptr(vel), n, _ptr(k_sum), float(T_desired), int(it),
                                              _ptr(t_out), self._stream()), "mdf_thermostat_scale")
 
     def density_modes(self, pos, q, out=None):
         """n(q) = sum_j exp(-i q x_j) per axis: complex128 tensor (nq, 3)."""
         n = pos.shape[0]
+        _need(pos, "pos", (n, 3), self.device)
         q = q.to(self.device, torch.float64).contiguous()
         nq = q.shape[0]
         if out is None:
             out = torch.empty((nq, 3), dtype=torch.complex128, device=self.device)
+        _need(out, "out", (nq, 3), self.device, torch.complex128)
         check(self._lib.mdf_density_modes(self._ctx, _ptr(pos), n, _ptr(q), nq, _ptr(out), self._stream()),
               "mdf_density_modes")
         return out
@@ -237,10 +257,14 @@ class P3M:
         grid q_m = m * q_step, m = 1..nq, when ``q_step`` is given.
         """
         n = pos.shape[0]
+        nsteps = int(nsteps)
         for t, name in ((pos, "pos"), (vel, "vel"), (acc, "acc")):
             _need(t, name, (n, 3), self.device)
+        if Z is not None:
+            _need(Z, "Z", (n,), self.device)
         if energies is None:
             energies = torch.empty((nsteps, 4), dtype=torch.float64, device=self.device)
+        _need(energies, "energies", (nsteps, 4), self.device)
         nqv = 0
         step = 0.0
         if q_step is not None:
@@ -252,6 +276,8 @@ class P3M:
             nqv = q.shape[0]
         if nqv and nqt is None:
             nqt = torch.empty((nsteps, nqv, 3), dtype=torch.complex128, device=self.device)
+        if nqv:
+            _need(nqt, "nqt", (nsteps, nqv, 3), self.device, torch.complex128)
         check(self._lib.mdf_md_run(self._ctx, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(Z), n, int(nsteps), float(dt),
                                    float(af), float(uf), float(kf), _ptr(q), nqv, step, _ptr(energies),
                                    _ptr(nqt) if nqv else None, self._stream()), "mdf_md_run")

@@ -87,3 +87,21 @@ def test_create_rejects_too_many_cells_before_allocating():
     assert b"too many cells" in msg and b"increase rc" in msg
     assert b"sub-cells" in msg
     assert not ctx.value
+
+
+def test_sqw_rejects_calls_beyond_the_grid_limit():
+    # 2^31 series samples or more are refused before any device call (the
+    # pointers are placeholders that are never dereferenced)
+    from mdforce._lib import lib
+    L = lib()
+    fake = ct.c_void_p(0x1000)
+    big = ct.c_size_t(1 << 62)
+    # 3 * 40000 series of 20000 samples = 2.4e9 > 2^31
+    rc = L.mdf_sqw(fake, 1, 20000, 40000, 1, 40000, 1.0, 0.1, 1.0, 1, fake, fake, big, None)
+    assert rc == -1 and b"exceed" in L.mdf_last_error()
+    # the workspace is two halves of 16 B per series sample; a null workspace is refused
+    nbytes = ct.c_size_t()
+    assert L.mdf_sqw_workspace(1, 20000, 40000, ct.byref(nbytes)) == 0
+    assert nbytes.value == 2 * 3 * 40000 * 20000 * 16
+    assert L.mdf_sqw(fake, 1, 20000, 40000, 1, 40000, 1.0, 0.1, 1.0, 1, fake, None, big, None) == -1
+    assert b"null workspace" in L.mdf_last_error()
