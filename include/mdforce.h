@@ -251,6 +251,19 @@ int mdf_sqw_workspace(int nrun, int npd, int nq, size_t *bytes);
 int mdf_sqw(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N, double dt,
             double T, int mean_axes, double *out, void *work, size_t work_bytes, void *stream);
 
+/* Allocation-free S(q, omega) (no counterpart in the reference): mdf_sqw for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as mdf_sqw without the workspace: the stages run in a static device
+ * array of MDF_SQW_STATIC_BYTES that the library holds on each GPU it runs on
+ * until the process ends, so a call takes at most MDF_SQW_STATIC_BYTES / 32
+ * series samples (nrun * nq * 3 * npd); larger calls return MDF_EINVAL.
+ * Calls no hipMalloc/hipFree.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+#define MDF_SQW_STATIC_BYTES (128u << 20)
+int mdf_sqw_noalloc(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N,
+                    double dt, double T, int mean_axes, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,8 @@
 // directly; the others ping-pong between two workspace halves.  A last kernel
 // forms |dt X|^2 / (N T) and, on request, the mean over the axes.
 // Memory-bound: each stage streams the series once in and once out.
+#include <mutex>
+
 #include "mdf_internal.h"
 
 namespace mdf {
@@ -205,6 +207,63 @@ extern "C" int mdf_sqw(const double *nqt, int nrun, int npd, int nq_total, int q
     }
     return sqw_run(nqt, nrun, npd, nq_total, q_stride, nq, N, dt, T, mean_axes, out, work, work_bytes,
                    (hipStream_t)stream);
+}
+
+namespace mdf {
+
+// ---- allocation-free form ----
+// The stages' ping-pong buffers in a library-owned device array: each GPU the
+// library runs on holds its own copy of MDF_SQW_STATIC_BYTES until the process
+// ends.  Host code never passes the array's name to a kernel or copy (on the
+// host it is a stand-in, not a device address): the address comes from
+// hipGetSymbolAddress, taken once per device with that device current, and
+// kept per device.
+static __device__ __align__(256) unsigned char scratch[MDF_SQW_STATIC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out) {
+    int dev = -1;
+    MDF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("mdf_sqw_noalloc: device %d beyond the %d this library tracks", dev, kMaxDevices);
+        return MDF_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("mdf_sqw_noalloc: hipGetSymbolAddress of the static scratch failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return MDF_EHIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MDF_OK;
+}
+}  // namespace
+
+}  // namespace mdf
+
+extern "C" int mdf_sqw_noalloc(const double *nqt, int nrun, int npd, int nq_total, int q_stride, int nq, double N,
+                               double dt, double T, int mean_axes, double *out, void *stream) {
+    size_t need = 0;
+    MDF_TRY(mdf_sqw_workspace(nrun, npd, nq, &need));
+    if (need > (size_t)MDF_SQW_STATIC_BYTES) {
+        set_error("mdf_sqw_noalloc: %zu bytes of scratch needed, the static scratch holds %zu "
+                  "(at most %zu series samples a call; use mdf_sqw with a workspace)",
+                  need, (size_t)MDF_SQW_STATIC_BYTES, (size_t)MDF_SQW_STATIC_BYTES / 32);
+        return MDF_EINVAL;
+    }
+    void *work = nullptr;
+    MDF_TRY(scratch_address(&work));
+    return sqw_run(nqt, nrun, npd, nq_total, q_stride, nq, N, dt, T, mean_axes, out, work,
+                   (size_t)MDF_SQW_STATIC_BYTES, (hipStream_t)stream);
 }
 
 namespace mdf {

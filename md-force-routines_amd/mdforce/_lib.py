@@ -25,6 +25,7 @@ class MdforceError(RuntimeError):
 
 
 MDF_EINVAL, MDF_ENOMEM, MDF_EHIP, MDF_ESTATE = -1, -2, -3, -4
+SQW_STATIC_BYTES = 128 << 20  # MDF_SQW_STATIC_BYTES
 
 
 class P3MParams(ct.Structure):
@@ -89,6 +90,7 @@ SIGNATURES = {
     "mdf_p3m_pm_error_sum": [_vp, _i, _i, _i, _i, _vp, _vp],
     "mdf_sqw_workspace": [_i, _i, _i, ct.POINTER(ct.c_size_t)],
     "mdf_sqw": [_vp, _i, _i, _i, _i, _i, _d, _d, _d, _i, _vp, _vp, ct.c_size_t, _vp],
+    "mdf_sqw_noalloc": [_vp, _i, _i, _i, _i, _i, _d, _d, _d, _i, _vp, _vp],
 }
 
 _lib = None
