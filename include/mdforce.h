@@ -144,6 +144,23 @@ int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const do
  * bin, pp, assign, fft, gather, drift, modes, finalize; resets the totals. */
 int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout);
 
+/* Verlet pair lists for the PP stage (no reference counterpart: the reference
+ * rebuilds its cell list and tests every candidate pair on every call,
+ * yukawa_pp.py:44-112).  skin > 0: the context's cells become
+ * L / floor(L / (rc + skin)) wide (rc + skin <= L/3 per axis), and a step
+ * tests candidates only when some particle has moved more than skin / 2
+ * since the last test ("build"), writing every pair within rc + skin to
+ * per-particle lists; the other steps evaluate the listed pairs with the exact
+ * cutoff r < rc.  Forces and energies are those of skin = 0 up to summation
+ * order.  mdf_p3m_force, mdf_pp_force and the first step of each mdf_md_run
+ * always build.  skin = 0 turns the lists off.  The context is rebuilt in
+ * place (the Green function and an attached network carry over; stage
+ * profiling stops).  Replica batches step without lists. */
+int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin);
+/* host pointer, 4 ints: list builds and list-evaluation steps since the
+ * skin was set, list entries per particle, reference cells along x. */
+int mdf_p3m_list_stats(mdf_p3m_ctx *ctx, int *out4);
+
 /* Diagnostics: number of PP tiles that needed more than one LDS pass and
  * particles whose stencil fell outside their LDS brick (both handled, only
  * slower), accumulated since create.  host pointer, 2 ints. */

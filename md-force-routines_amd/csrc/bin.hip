@@ -45,6 +45,7 @@ struct BinArgs {
     int *cell_start;
     double4 *spos;
     int *perm;
+    int *inv_perm;     // caller index -> sorted slot, or null
 };
 
 __device__ __forceinline__ void bin_count_body(const BinArgs &A, const Blk &b) {
@@ -181,6 +182,7 @@ __device__ __forceinline__ void bin_place_body(const BinArgs &A, const Blk &b) {
     const double z = wrap_box(pos[3 * i + 2], g.L[2]);
     spos[s] = make_double4(x, y, z, Z ? Z[i] : 1.0);
     perm[s] = i;
+    if (A.inv_perm) A.inv_perm[i] = s;
 }
 
 __global__ void k_bin_place(BinArgs A) { bin_place_body(A, blk_plain()); }
@@ -235,6 +237,7 @@ static BinArgs bin_args(mdf_p3m_ctx *c, const double *pos, const double *Z, int 
     A.cell_start = c->cell_start;
     A.spos = c->spos;
     A.perm = c->perm;
+    A.inv_perm = c->inv_perm;
     return A;
 }
 

@@ -87,6 +87,9 @@ static void free_ctx(mdf_p3m_ctx *c) {
                     c->twf[2],    c->part_pp,   c->part_pm,  c->part_red,   c->scal,       c->stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->inv_perm) (void)hipFree(c->inv_perm);
+    if (c->stab) (void)hipFree(c->stab);
+    vlist_free(c);
     delete c;
 }
 
@@ -137,12 +140,24 @@ int mdf_abi_version(void) { return MDF_ABI_VERSION; }
 
 const char *mdf_last_error(void) { return g_err; }
 
-int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
+}  // extern "C"
+
+namespace mdf {
+
+// A context whose cell grid is sized for rc + skin (skin = 0: the reference's
+// cells of L / floor(L / rc), yukawa_pp.py:17-23); the pair terms and the
+// cutoff stay rc.  skin > 0 also allocates the Verlet pair lists.
+static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
     if (!p || !out) {
         set_error("mdf_p3m_create: null argument");
         return MDF_EINVAL;
     }
     *out = nullptr;
+    if (!(skin >= 0.0) || !(skin < 1e300)) {
+        set_error("pair-list skin %g: need a finite skin >= 0", skin);
+        return MDF_EINVAL;
+    }
+    const double rcg = p->rc + skin;  // the cells hold every pair within rc + skin
     if (p->order != 6) {
         set_error("charge-assignment order %d not supported (only 6, as yukawa_pm_ord6)", p->order);
         return MDF_EINVAL;
@@ -168,8 +183,12 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
             set_error("box[%d] must be positive", d);
             return MDF_EINVAL;
         }
-        if ((int)floor(p->box[d] / p->rc) < 3) {
-            set_error("rc = %g too large for box %g: the cell list needs L/rc >= 3 per axis", p->rc, p->box[d]);
+        if ((int)floor(p->box[d] / rcg) < 3) {
+            if (skin > 0.0)
+                set_error("rc + skin = %g too large for box %g: the cell list needs L/(rc + skin) >= 3 per axis", rcg,
+                          p->box[d]);
+            else
+                set_error("rc = %g too large for box %g: the cell list needs L/rc >= 3 per axis", p->rc, p->box[d]);
             return MDF_EINVAL;
         }
     }
@@ -177,7 +196,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         // cell counts (reference cells floor(L/rc) per axis, two sub-cells each),
         // refused before anything is allocated
         double nb = 1.0;
-        for (int d = 0; d < 3; ++d) nb *= floor(p->box[d] / p->rc);
+        for (int d = 0; d < 3; ++d) nb *= floor(p->box[d] / rcg);
         const double ns = 8.0 * nb;
         if (ns > 1024.0 * 8192.0) {
             set_error("too many cells (%.0f sub-cells > %d): increase rc", ns, 1024 * 8192);
@@ -201,7 +220,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         g.M[d] = p->mesh[d];
         g.h[d] = p->box[d] / p->mesh[d];
         g.ih[d] = 1.0 / g.h[d];
-        const int lxd = (int)floor(p->box[d] / p->rc);  // reference cells, yukawa_pp.py:17-19
+        const int lxd = (int)floor(p->box[d] / rcg);  // reference cells, yukawa_pp.py:17-19
         g.nsub[d] = 2 * lxd;
         g.nblk[d] = lxd;
         g.w[d] = p->box[d] / g.nsub[d];
@@ -219,16 +238,20 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
 
     // LDS tile capacity of the PP kernel: mean tile + 6 sigma (Poisson) + slack;
     // larger tiles take more than one pass
+    // (with pair lists 4 sigma + 32: a build whose tile overflows only builds
+    // again on the next step, and the smaller tile keeps two PP workgroups on
+    // a CU at the larger cells of rc + skin)
     const double mean_tile = 216.0 * (double)p->n_max / (double)c->ncell;
-    double tm = mean_tile + 6.0 * sqrt(mean_tile) + 64.0;
+    double tm = skin > 0.0 ? mean_tile + 4.0 * sqrt(mean_tile) + 32.0 : mean_tile + 6.0 * sqrt(mean_tile) + 64.0;
     const double tcap = (150.0 * 1024.0 - 5120.0) / (mixed ? 32.0 : 24.0);
     if (tm > tcap) tm = tcap;
     if (tm < 512.0) tm = 512.0;
     c->tile_max = ((int)tm + 3) & ~3;  // keeps the table after the tile 16-byte aligned
-    std::vector<double2> pair_tab;
+    std::vector<double2> pair_tab, s_tab;
     {
-        const int rr = mixed ? build_pair_table(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, true, pair_tab, c->npiece)
-                             : build_pair_table(p->kappa, p->alpha, p->rc, kChebDeg, 4e-15, false, pair_tab, c->npiece);
+        int rr = mixed ? build_pair_table(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, true, pair_tab, c->npiece)
+                       : build_pair_table(p->kappa, p->alpha, p->rc, kChebDeg, 4e-15, false, pair_tab, c->npiece);
+        if (rr == MDF_OK && mixed) rr = build_pair_table_s(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, s_tab, c->sp);
         if (rr != MDF_OK) {
             delete c;
             return rr;
@@ -246,7 +269,7 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     if (mixed) {
         int best = 16, best_waves = 0;
         for (int nw : {8, 12, 16}) {
-            const size_t l = pp_lds_bytes(c->tile_max, c->npiece, true, nw) + 1024;
+            const size_t l = pp_lds_bytes(c->tile_max, pp_table_pieces(c), true, nw) + 1024;
             const int wg = (int)std::min<size_t>((160 * 1024) / l, (size_t)(24 / nw));
             if (wg * nw > best_waves) {
                 best_waves = wg * nw;
@@ -259,21 +282,21 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         // that two workgroups of 12 (80 VGPR, see k_pp) when their LDS fits,
         // else one of 16.  At rc 5.5 the 4-wave layout beat 12 waves with
         // the spill (1.69 against 1.78 ms), so 12 replaces 16 only.
-        const bool big = pp_lds_bytes(c->tile_max, c->npiece, false, 4) > 64 * 1024;
-        const bool two12 = 2 * (pp_lds_bytes(c->tile_max, c->npiece, false, 12) + 1536) <= 160 * 1024;
+        const bool big = pp_lds_bytes(c->tile_max, pp_table_pieces(c), false, 4) > 64 * 1024;
+        const bool two12 = 2 * (pp_lds_bytes(c->tile_max, pp_table_pieces(c), false, 12) + 1536) <= 160 * 1024;
         c->pp_waves = !big ? 4 : (two12 ? 12 : 16);
     }
-    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
+    c->pp_lds = pp_lds_bytes(c->tile_max, pp_table_pieces(c), mixed, c->pp_waves);
     // dynamic LDS plus the kernel's static arrays (< 4 KB) must fit 160 KB, for
     // this wave count and for that of segmented launches (16 above 64 KB)
     const size_t lds_cap = 160 * 1024 - 4096;
     auto lds_need = [&]() {
-        return std::max(pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves),
-                        pp_lds_bytes(c->tile_max, c->npiece, mixed, 16));
+        return std::max(pp_lds_bytes(c->tile_max, pp_table_pieces(c), mixed, c->pp_waves),
+                        pp_lds_bytes(c->tile_max, pp_table_pieces(c), mixed, 16));
     };
     while (lds_need() > lds_cap && c->tile_max > 0)
         c->tile_max -= (int)(((lds_need() - lds_cap) / 32 + 4) & ~(size_t)3);
-    c->pp_lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, c->pp_waves);
+    c->pp_lds = pp_lds_bytes(c->tile_max, pp_table_pieces(c), mixed, c->pp_waves);
     if (c->tile_max < 256) {
         delete c;
         set_error("PP pair table too large for LDS (kappa=%g alpha=%g rc=%g)", p->kappa, p->alpha, p->rc);
@@ -351,12 +374,14 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     ALLOC(c->part_cell, (size_t)p->n_max);
     ALLOC(c->part_slot, (size_t)p->n_max);
     ALLOC(c->perm, (size_t)p->n_max);
+    ALLOC(c->inv_perm, (size_t)p->n_max);
     ALLOC(c->spos, (size_t)p->n_max);
     ALLOC(c->acc_pp_s, 3 * (size_t)p->n_max);
     ALLOC(c->rho, c->nmesh);
     ALLOC(c->green, c->nspec);
     if (mixed) {
         ALLOC(c->chebf, pair_tab.size());
+        ALLOC(c->stab, s_tab.size());
         ALLOC(c->ckf, c->nspec);
         ALLOC(c->ekf, 3 * c->nspec);
         ALLOC(c->emeshf, 3 * c->nmesh);
@@ -384,6 +409,11 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         std::vector<float2> tf(pair_tab.size());
         for (size_t i = 0; i < tf.size(); ++i) tf[i] = make_float2((float)pair_tab[i].x, (float)pair_tab[i].y);
         up = hipMemcpy(c->chebf, tf.data(), tf.size() * sizeof(float2), hipMemcpyHostToDevice);
+        if (up == hipSuccess) {
+            std::vector<float2> ts(s_tab.size());
+            for (size_t i = 0; i < ts.size(); ++i) ts[i] = make_float2((float)s_tab[i].x, (float)s_tab[i].y);
+            up = hipMemcpy(c->stab, ts.data(), ts.size() * sizeof(float2), hipMemcpyHostToDevice);
+        }
     } else {
         up = hipMemcpy(c->cheb, pair_tab.data(), pair_tab.size() * sizeof(double2), hipMemcpyHostToDevice);
     }
@@ -416,6 +446,18 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
         free_ctx(c);
         return r;
     }
+    // the lists take the largest skin the cells allow: every pair within the
+    // smallest cell width (L / cells, at least rc + skin) is inside the stencil
+    double skin_eff = 0.0;
+    if (skin > 0.0) {
+        double wmin = 2.0 * g.w[0];
+        for (int d = 1; d < 3; ++d) wmin = std::min(wmin, 2.0 * g.w[d]);
+        skin_eff = wmin - p->rc;
+    }
+    if ((r = vlist_alloc(c, skin_eff)) != MDF_OK) {
+        free_ctx(c);
+        return r;
+    }
     if (hipDeviceSynchronize() != hipSuccess) {
         set_error("device synchronisation failed after create");
         free_ctx(c);
@@ -424,6 +466,62 @@ int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     *out = c;
     return MDF_OK;
 }
+
+}  // namespace mdf
+
+extern "C" {
+
+int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) { return create_ctx(p, 0.0, out); }
+
+int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin) {
+    if (!ctx) {
+        set_error("mdf_p3m_set_skin: null context");
+        return MDF_EINVAL;
+    }
+    if (skin == ctx->vl_skin) return MDF_OK;
+    // a new context with the cells of rc + skin; the Green function and the
+    // attached network carry over, then the two swap contents so that the
+    // caller's handle stays valid
+    mdf_p3m_ctx *nc = nullptr;
+    MDF_TRY(create_ctx(&ctx->prm, skin, &nc));
+    if (ctx->have_green) {
+        hipError_t e = hipMemcpy(nc->green, ctx->green, ctx->nspec * sizeof(double), hipMemcpyDeviceToDevice);
+        if (e == hipSuccess && ctx->greenf)
+            e = hipMemcpy(nc->greenf, ctx->greenf, ctx->nspec * sizeof(float), hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) {
+            free_ctx(nc);
+            set_error("mdf_p3m_set_skin: Green function copy failed (%s)", hipGetErrorString(e));
+            return MDF_EHIP;
+        }
+        nc->have_green = true;
+    }
+    nc->mlp = ctx->mlp;
+    (void)hipDeviceSynchronize();
+    mdf_p3m_ctx tmp;
+    std::memcpy((void *)&tmp, (void *)ctx, sizeof tmp);
+    std::memcpy((void *)ctx, (void *)nc, sizeof tmp);
+    std::memcpy((void *)nc, (void *)&tmp, sizeof tmp);
+    free_ctx(nc);
+    return MDF_OK;
+}
+
+int mdf_p3m_list_stats(mdf_p3m_ctx *ctx, int *out4) {
+    if (!ctx || !out4) {
+        set_error("mdf_p3m_list_stats: null argument");
+        return MDF_EINVAL;
+    }
+    int h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ctx->vl_flags) MDF_HIP(hipMemcpy(h, ctx->vl_flags, sizeof h, hipMemcpyDeviceToHost));
+    out4[0] = h[4];  // builds
+    out4[1] = h[5];  // steps on the lists
+    out4[2] = ctx->vl_cap;
+    out4[3] = ctx->g.nblk[0];
+    return MDF_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int mdf_p3m_destroy(mdf_p3m_ctx *ctx) {
     if (!ctx) return MDF_OK;
@@ -470,6 +568,8 @@ int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, d
     hipStream_t s = (hipStream_t)stream;
     prof_mark(ctx, kStageBin, s);
     MDF_TRY(bin_particles(ctx, pos, Z, n, s));
+    ctx->vl_force_build = 1;  // positions from the caller: build the lists anew
+    MDF_TRY(vlist_prepare(ctx, pos, n, s));
     prof_mark(ctx, kStagePP, s);
     MDF_TRY(pp_forces(ctx, n, s));
     if (ctx->mlp) MDF_TRY(mlp_correct(ctx->mlp, ctx, n, s));
@@ -502,6 +602,8 @@ int mdf_pp_force(mdf_p3m_ctx *ctx, const double *pos, int n, double *acc_pp, dou
     MDF_TRY(check_n(ctx, n));
     hipStream_t s = (hipStream_t)stream;
     MDF_TRY(bin_particles(ctx, pos, nullptr, n, s));
+    ctx->vl_force_build = 1;
+    MDF_TRY(vlist_prepare(ctx, pos, n, s));
     MDF_TRY(pp_forces(ctx, n, s));
     if (acc_pp) MDF_TRY(unpermute3(ctx, ctx->acc_pp_s, acc_pp, n, s));
     MDF_TRY(finalize_energy(ctx, n, 1.0, 1, 0, nullptr, s));
@@ -584,6 +686,7 @@ int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const do
     MDF_TRY(check_n(ctx, n));
     MDF_TRY(need_green(ctx));
     hipStream_t s = (hipStream_t)stream;
+    ctx->vl_force_build = 1;  // the caller may have moved the particles since the last call
     for (int it = 0; it < nsteps; ++it) {
         MDF_TRY(md_step(ctx, pos, vel, acc, Z, n, dt, af, uf, kf, energies ? energies + 4 * (size_t)it : nullptr, s));
         if (nq > 0) {

@@ -66,6 +66,16 @@ struct Brick {
     int lds_gather;    // 1 if the 3-field brick fits LDS for the gather
 };
 
+// FP32 pair table indexed by the float bits of s = r^2 (pair_table.hip): piece
+// p = (bits(s) >> shift) - base for s >= s_lo, t = tscale * f + toff with f the
+// float of s's low `shift` mantissa bits under the exponent of 1.
+struct SPieces {
+    int npiece;
+    unsigned base;
+    int shift;
+    float s_lo, tscale, toff;
+};
+
 constexpr int kPMThreads = 512;
 constexpr int kGatherThreads = 1024;
 constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces (FP64)
@@ -103,6 +113,8 @@ struct mdf_p3m_ctx {
     double2 *cheb;      // PP pair-term tables, npiece * (kChebDeg + 1) (g1, g2) pairs
     int npiece;
     float2 *chebf;      // FP32 tables of the mixed mode, npiece * (kChebDegF + 1)
+    float2 *stab;       // FP32 s-indexed pair table of the mixed mode, sp.npiece * (kChebDegF + 1)
+    mdf::SPieces sp;
 
     // mesh
     double *rho;        // nmesh
@@ -127,6 +139,24 @@ struct mdf_p3m_ctx {
     int *stats;         // device Stats
     struct Prof *prof;  // optional stage timing (null when off)
     mdf_mlp *mlp;       // attached learned PP correction (null = none)
+
+    // Verlet pair lists of the PP stage (vl_skin > 0: on; mdf_p3m_set_skin).
+    // Lists hold, per particle, the tile slots of every j within rc + skin at
+    // the last build, in the frozen order (sorted order and cell starts of that
+    // build); they stay valid while no particle has moved more than skin / 2.
+    int *inv_perm;           // caller index -> current sorted slot (written by every binning)
+    double vl_skin;
+    int vl_cap;              // list entries per particle
+    unsigned short *vl_list; // n_max * vl_cap
+    int *vl_cnt;             // n_max
+    double4 *vl_pos;         // frozen order: this step's positions, unwrapped near vl_ref
+    double4 *vl_ref;         // frozen order: positions at the build
+    int *vl_perm;            // frozen slot -> caller index
+    int *vl_cs;              // frozen cell starts (ncell + 1)
+    int *vl_flags;           // [0] build requested, [1] last build incomplete, [3] mode of the step
+                             // (1 build, 2 lists), [4] builds, [5] list steps (counts)
+    int vl_force_build;      // host: the next list-mode step must build
+    int vl_n;                // host: particle count of the last step prepared (a change builds)
 };
 
 namespace mdf {
@@ -175,7 +205,12 @@ int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
 // PP pair-term tables (host) and the PP kernel's dynamic LDS size.
 int build_pair_table(double kappa, double alpha, double rc, int deg, double tol, bool as_float,
                      std::vector<double2> &tab, int &npiece);
+int build_pair_table_s(double kappa, double alpha, double rc, int deg, double tol, std::vector<double2> &tab,
+                       SPieces &sp);
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw);
+// pieces of the PP table staged in LDS: the s-indexed table (FP32) or the
+// r-indexed Chebyshev table (FP64)
+inline int pp_table_pieces(const mdf_p3m_ctx *c);
 // FP32 copy of the FP64 half-spectrum Green function (mixed mode).
 int green_to_float(mdf_p3m_ctx *c, hipStream_t s);
 // Learned correction of the attached network, added to acc_pp_s.
@@ -205,6 +240,14 @@ int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int 
                        hipStream_t s);
 // Copy a sorted (n,3) array back to the caller's particle order.
 int unpermute3(mdf_p3m_ctx *c, const double *src_sorted, double *dst, int n, hipStream_t s);
+
+// Verlet lists (vlist.hip): after the binning of a step, decide whether the
+// lists still hold (no particle moved more than skin / 2 since the build, no
+// incomplete build, no build requested), refresh the frozen copies when they
+// do not, and set the mode the PP kernel reads.  pos: caller order.
+int vlist_prepare(mdf_p3m_ctx *c, const double *pos, int n, hipStream_t s);
+int vlist_alloc(mdf_p3m_ctx *c, double skin);
+void vlist_free(mdf_p3m_ctx *c);
 
 // Dynamic LDS above 64 KiB (gfx950 has 160 KiB per workgroup) must be
 // allowed per kernel before launch.
@@ -278,6 +321,10 @@ void seg_stages_bin(std::vector<SegStage> &out);
 void seg_stages_pp(std::vector<SegStage> &out, int precision);
 void seg_stages_pm(std::vector<SegStage> &out, int precision, int part);  // 0: fill + assign, 1: gather
 void seg_stages_fft(std::vector<SegStage> &out, int precision);
+
+inline int pp_table_pieces(const mdf_p3m_ctx *c) {
+    return c->prm.precision == MDF_PREC_MIXED ? c->sp.npiece : c->npiece;
+}
 
 // Wave (64-lane) helpers.
 __device__ __forceinline__ double wave_sum(double v) {

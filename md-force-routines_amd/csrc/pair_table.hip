@@ -20,6 +20,7 @@
 // (N = 1e6): fewer FMAs and table reads per pair beat the larger table
 // down to degree 4.
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "mdf_internal.h"
@@ -137,6 +138,111 @@ int build_pair_table(double kappa, double alpha, double rc, int deg, double tol,
         }
     }
     set_error("pair-term table did not reach %g with 512 pieces (kappa=%g alpha=%g rc=%g)", tol, kappa, alpha, rc);
+    return MDF_EINVAL;
+}
+
+}  // namespace mdf
+
+namespace mdf {
+
+// FP32 table in s = r^2 for the mixed-precision PP kernel.  A piece is 2^-m
+// of an octave of s, so the float bits of s give the piece directly: its
+// exponent and top m mantissa bits, counted from s_lo = 2^-2 (r = 1/2); the
+// low 23 - m mantissa bits give the position t in the piece.  Pieces of equal
+// relative width follow the r^-1 and r^-3 growth of the pair terms at small r
+// with one degree, and no square root is needed to find a pair's piece.  Per
+// piece the table holds, as degree-deg power series of t in [-1, 1),
+//   phi(s) = g1(r) / r      (pair energy)
+//   F(s)   = g2(r) / r^3    (force on i: F (x_i - x_j))
+// fitted at the Chebyshev nodes in long double; the fit with coefficients
+// rounded to float is checked at 64 points per piece, with the same measure as
+// the r-indexed table: the errors of g1 = r phi and g2 = r^3 F against tol
+// times max|g1| and max|g2| on [0, rc]; m grows (to at most 6) until it
+// passes.  Pairs closer than 1/2 (s < s_lo) use the r-indexed table instead.
+int build_pair_table_s(double kappa, double alpha, double rc, int deg, double tol, std::vector<double2> &tab,
+                       SPieces &sp) {
+    const int nc = deg + 1;
+    if (nc > 32) {
+        set_error("pair-table degree %d too large", deg);
+        return MDF_EINVAL;
+    }
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const float s_lo = 0.25f;
+    for (int m = 3; m <= 6; ++m) {
+        const int shift = 23 - m;
+        unsigned lo_bits;
+        std::memcpy(&lo_bits, &s_lo, 4);
+        const unsigned base = lo_bits >> shift;
+        const float s_top = (float)(rc * rc) * (1.0f + 1e-6f);
+        unsigned top_bits;
+        std::memcpy(&top_bits, &s_top, 4);
+        const int P = (int)((top_bits >> shift) - base) + 1;
+        tab.assign((size_t)P * nc, make_double2(0.0, 0.0));
+        auto piece_s = [&](int p, long double t) {  // s at t in [-1, 1] of piece p
+            unsigned b0 = (base + (unsigned)p) << shift, b1 = (base + (unsigned)p + 1) << shift;
+            float f0, f1;
+            std::memcpy(&f0, &b0, 4);
+            std::memcpy(&f1, &b1, 4);
+            return (long double)f0 + (t + 1.0L) * 0.5L * ((long double)f1 - (long double)f0);
+        };
+        auto funcs = [&](long double s, long double &phi, long double &F) {
+            const long double r = sqrtl(s);
+            long double g1, g2;
+            pair_g(r, kappa, alpha, g1, g2);
+            phi = g1 / r;
+            F = g2 / (r * s);
+        };
+        long double g1max = 0.0L, g2max = 0.0L;  // on [0, rc], as the r-indexed table's check
+        for (int q = 0; q <= 4096; ++q) {
+            long double g1, g2;
+            pair_g((long double)rc * q / 4096.0L, kappa, alpha, g1, g2);
+            g1max = fmaxl(g1max, fabsl(g1));
+            g2max = fmaxl(g2max, fabsl(g2));
+        }
+        bool ok = true;
+        for (int p = 0; p < P && ok; ++p) {
+            long double v1[32], v2[32];
+            for (int j = 0; j < nc; ++j) funcs(piece_s(p, cosl(pi * (j + 0.5L) / nc)), v1[j], v2[j]);
+            long double c1[32], c2[32];
+            for (int k = 0; k < nc; ++k) {
+                long double s1 = 0.0L, s2 = 0.0L;
+                for (int j = 0; j < nc; ++j) {
+                    const long double cs = cosl(pi * k * (j + 0.5L) / nc);
+                    s1 += v1[j] * cs;
+                    s2 += v2[j] * cs;
+                }
+                const long double f = (k == 0 ? 1.0L : 2.0L) / nc;
+                c1[k] = f * s1;
+                c2[k] = f * s2;
+            }
+            long double m1[32], m2[32];
+            cheb_to_power(c1, deg, m1);
+            cheb_to_power(c2, deg, m2);
+            double2 *c = &tab[(size_t)p * nc];
+            for (int k = 0; k < nc; ++k) c[k] = make_double2((double)m1[k], (double)m2[k]);
+            long double e1 = 0.0L, e2 = 0.0L;
+            for (int q = 0; q <= 64; ++q) {
+                const long double t = -1.0L + 2.0L * q / 64.0L;
+                const long double sv = piece_s(p, t), r = sqrtl(sv);
+                long double a, b;
+                funcs(sv, a, b);
+                e1 = fmaxl(e1, fabsl(horner_ld(c, deg, 0, t) - a) * r);
+                e2 = fmaxl(e2, fabsl(horner_ld(c, deg, 1, t) - b) * r * sv);
+            }
+            ok = e1 <= (long double)tol * g1max && e2 <= (long double)tol * g2max;
+        }
+        if (ok) {
+            sp.npiece = P;
+            sp.base = base;
+            sp.shift = shift;
+            sp.s_lo = s_lo;
+            sp.tscale = (float)(1 << (m + 1));
+            sp.toff = -(float)((1 << (m + 1)) + 1);
+            return MDF_OK;
+        }
+    }
+    set_error("s-indexed pair table did not reach %g with 64 pieces an octave (kappa=%g alpha=%g rc=%g)", tol, kappa,
+              alpha, rc);
     return MDF_EINVAL;
 }
 

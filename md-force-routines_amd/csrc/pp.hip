@@ -74,10 +74,32 @@ struct PPArgs {
     double *__restrict__ part;         // per-block energy partial
     int *__restrict__ stats;
     int tile_max;
-    int npiece;
+    int npiece;                        // pieces of the r-indexed table
+    int ntab;                          // pieces of the table staged in LDS (cheb)
     T piece_w, inv_piece_w;
+    // FP32: cheb is the s-indexed table (SPieces); the r-indexed table, for
+    // pairs closer than s_lo, is read from global memory
+    const T2 *__restrict__ rtab;
+    unsigned s_base;
+    int s_shift;
+    float s_lo, s_tscale, s_toff;
     T rc2;                             // rc^2, the pair cutoff
     T rc2m;                            // candidate test radius^2 (rc slightly widened)
+    // Verlet pair lists (null vl_flags: off).  vl_flags[3] is the step's mode:
+    // 1 = build (the sweep tests candidates within the list radius, evaluates
+    // those within rc and writes every candidate it queued to the list of its
+    // i), 2 = evaluate the lists.  A build that cannot list a block (a tile in
+    // several passes, a list over vl_cap) sets vl_flags[1], which makes the
+    // next step build again.  spos and cell_start are then the frozen copies
+    // the lists index; the force of frozen slot s goes to current sorted slot
+    // inv_perm[vl_perm[s]].
+    int *vl_flags;
+    unsigned short *__restrict__ vl_list;  // vl_cap entries per frozen slot: tile slots of its j
+    int *__restrict__ vl_cnt;               // entries per frozen slot
+    const int *__restrict__ vl_perm;        // frozen slot -> caller index
+    const int *__restrict__ inv_perm;       // caller index -> current sorted slot
+    int vl_cap;
+    T rcl2;                                 // list radius^2: (rc + skin)^2, slightly widened
 };
 
 template <class T>
@@ -167,6 +189,73 @@ __device__ __forceinline__ void pp_pair_cheb(const PPArgs<T> &P, const typename 
     fz += fr * dz;
 }
 
+// The same pair terms without branches: a pair that is not ok, outside the
+// cutoff or at r = 0 is evaluated at r = 1 and contributes nothing (selects,
+// not multiplications, so no inf or NaN leaks in).  Two of these in a row
+// have no control dependence, so their long latency chains interleave.  The
+// arithmetic of a contributing pair is pp_pair_cheb's, operation for
+// operation.
+template <class T>
+__device__ __forceinline__ void pp_pair_masked(const PPArgs<T> &P, const typename Vec2<T>::t *__restrict__ tab,
+                                               bool ok, T dx, T dy, T dz, T &fx, T &fy, T &fz, T &u) {
+    constexpr int D = ChebDeg<T>::v;
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const bool in = ok && r2 < P.rc2 && r2 > (T)0;
+    const T r2s = in ? r2 : (T)1;
+    const T ir = rsqrt_t<T>(r2s);
+    const T r = r2s * ir;
+    const int p = min((int)(r * P.inv_piece_w), P.npiece - 1);
+    const T t = (r - p * P.piece_w) * ((T)2 * P.inv_piece_w) - (T)1;
+    using LdsT2 = __attribute__((address_space(3))) const typename Vec2<T>::t;
+    LdsT2 *c = (LdsT2 *)tab + __umul24((unsigned)p, D + 1);
+    T g1, g2;
+    if constexpr (sizeof(T) == 4) horner2<T>(c, t, g1, g2);
+    else cheb2<T>(c, t, g1, g2);
+    const T uu = in ? g1 * ir : (T)0;
+    const T fr = in ? g2 * ir * ir * ir : (T)0;
+    u += uu;
+    fx = fma(fr, dx, fx);
+    fy = fma(fr, dy, fy);
+    fz = fma(fr, dz, fz);
+}
+
+// FP32 pair terms from the s-indexed table (pair_table.hip): the piece and
+// its local variable come from the float bits of s = r^2, and the table holds
+// phi(s) and F(s) = f(r) / r directly, so a pair needs no square root.  Pairs
+// with s < s_lo (r < 1/2: the release of a random start, never the liquid)
+// take the r-indexed table from global memory.  Contributions of pairs that
+// are not ok, outside the cutoff or at r = 0 are selected away (no branch
+// except the rare close-pair one).
+__device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, const float2 *__restrict__ stab, bool ok, float dx,
+                                          float dy, float dz, float &fx, float &fy, float &fz, float &u) {
+    constexpr int D = kChebDegF;
+    const float s2 = dx * dx + dy * dy + dz * dz;
+    const bool in = ok && s2 < P.rc2 && s2 > 0.0f;
+    const unsigned b = __float_as_uint(s2);
+    int p = (int)(b >> P.s_shift) - (int)P.s_base;
+    p = min(max(p, 0), P.ntab - 1);
+    const float t = fmaf(__uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u), P.s_tscale, P.s_toff);
+    using LdsF2 = __attribute__((address_space(3))) const float2;
+    LdsF2 *c = (LdsF2 *)stab + __umul24((unsigned)p, D + 1);
+    float phi, F;
+    horner2<float>(c, t, phi, F);
+    if (in && s2 < P.s_lo) {
+        const float ir = rsqrt_t<float>(s2);
+        const float r = s2 * ir;
+        const int q = min((int)(r * P.inv_piece_w), P.npiece - 1);
+        const float tq = (r - q * P.piece_w) * (2.0f * P.inv_piece_w) - 1.0f;
+        float g1, g2;
+        horner2<float>(P.rtab + q * (D + 1), tq, g1, g2);
+        phi = g1 * ir;
+        F = g2 * ir * ir * ir;
+    }
+    u += in ? phi : 0.0f;
+    const float Fm = in ? F : 0.0f;
+    fx = fmaf(Fm, dx, fx);
+    fy = fmaf(Fm, dy, fy);
+    fz = fmaf(Fm, dz, fz);
+}
+
 __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, double &shift) {
     if (c < 0) {
         cw = c + n;
@@ -206,7 +295,7 @@ __device__ __forceinline__ float half_wave_sum(float v) {
 // NW waves per workgroup share the tile: 4 for tiles up to 64 KB, 16 for
 // larger ones (large rc), where one workgroup per CU would otherwise leave a
 // single wave per SIMD.
-template <class R, int NW>
+template <class R, int NW, bool LISTS>
 __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsigned char *ldsraw) {
     using R2 = typename Vec2<R>::t;
     constexpr int D = ChebDeg<R>::v;
@@ -226,7 +315,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     float4 *tl4 = th4 + T;                                 // mixed, unpacked: lo (x, y, z)
     float2 *tl2 = (float2 *)(th4 + T);                     // mixed, packed: lo (y, z)
     R2 *tab = kRel ? (kPack ? (R2 *)(tl2 + T) : (R2 *)(tl4 + T)) : (R2 *)(tz + T);  // npiece * (D + 1)
-    int *cum = (int *)(tab + P.npiece * (D + 1));          // 217 prefix offsets of the tile sub-cells
+    int *cum = (int *)(tab + P.ntab * (D + 1));            // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // NW * kGroups * kQCap
     __shared__ int tstart[216];  // sorted-array start of each tile sub-cell
     __shared__ int2 rng[8][5];   // per home sub-cell: LDS range of each of its 5 stencil planes
@@ -240,7 +329,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     const int by = (bid / g.nblk[0]) % g.nblk[1];
     const int bz = bid / (g.nblk[0] * g.nblk[1]);
 
-    for (int t = tid; t < P.npiece * (D + 1); t += blockDim.x) tab[t] = P.cheb[t];
+    for (int t = tid; t < P.ntab * (D + 1); t += blockDim.x) tab[t] = P.cheb[t];
     // origin of the tile coordinates: 0, or the home block centre (mixed)
     const double ox = kRel ? (2 * bx + 1) * g.w[0] : 0.0;
     const double oy = kRel ? (2 * by + 1) * g.w[1] : 0.0;
@@ -301,7 +390,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     // otherwise turn gmask & x into a select on grp (that needs x in a VGPR)
     unsigned gmask = grp ? ~0u : 0u;
     asm volatile("" : "+v"(gmask));
-    const R rc2m = P.rc2m;
+    // Verlet list mode of this step (uniform over the launch)
+    const int vmode = LISTS ? __builtin_amdgcn_readfirstlane(P.vl_flags[3]) : 0;
+    const bool build = vmode == 1, use_list = vmode == 2;
+    const R rc2m = build ? P.rcl2 : P.rc2m;
     double u_acc = 0.0;
 
     int r0 = 0, pass = 0;
@@ -314,7 +406,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             r0 = r0 + 1;
             continue;
         }
-        if (pass == 1 && tid == 0) atomicAdd(&P.stats[0], 1);
+        if (pass == 1 && tid == 0) {
+            atomicAdd(&P.stats[0], 1);
+            if (build) atomicOr(&P.vl_flags[1], 1);  // lists need the whole tile in one pass
+        }
         // 3. stage rows [r0, r1) with their image shifts
         for (int r = r0 + wave; r < r1; r += NW) {
             const int tys = r % 6, tzs = r / 6;
@@ -427,8 +522,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 }
             }
             R fx = 0, fy = 0, fz = 0, u = 0;
-            auto pair_at = [&](int jj) {
-                R dx, dy, dz;
+            auto disp = [&](int jj, R &dx, R &dy, R &dz) {
                 if constexpr (kRel) {
                     const float4 a = th4[jj];
                     if constexpr (kPack) {
@@ -447,10 +541,55 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                     dy = yi - ty[jj];
                     dz = zi - tz[jj];
                 }
-                pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
+            };
+            auto pair_at = [&](int jj) {
+                R dx, dy, dz;
+                disp(jj, dx, dy, dz);
+                if constexpr (kRel) pp_pair_f(P, tab, true, dx, dy, dz, fx, fy, fz, u);
+                else pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
             };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
-            for (int rr = 0; rr < 5; ++rr) {
+            int lc0 = 0, lc1 = 0;  // list entries written for the i of half 0 / half 1 (build)
+            unsigned short *const lst = build || use_list ? P.vl_list + (size_t)gi * P.vl_cap : nullptr;
+            // build: the queued j of this half go to its i's list, slots lc .. lc + cnt - 1
+            auto list_put = [&](int jj, int at) {
+                if (at < P.vl_cap) lst[at] = (unsigned short)jj;
+            };
+            if (use_list) {
+                // the step's pairs from i's list: every j within the list radius at
+                // the last build, evaluated with the exact cutoff.  A lane takes two
+                // consecutive entries (one dword) of each chunk of 2 kGL; the
+                // chunks are loaded two ahead of their use, so that the list reads
+                // from memory overlap the pair terms of the chunks before
+                const int cnt = have_i ? P.vl_cnt[gi] : 0;
+                const int nchunk = (max(readlane_i(cnt, 0), readlane_i(cnt, kGL)) + 2 * kGL - 1) / (2 * kGL);
+                const unsigned *lw = (const unsigned *)lst;  // vl_cap is a multiple of 32: dword aligned
+                // the load itself is unconditional (its index clamped into i's list)
+                // so that the wait for it can be placed where its value is used
+                const int wlast = P.vl_cap / 2 - 1;
+                auto chunk_word = [&](int c) -> unsigned { return lw[min(c * kGL + gl, wlast)]; };
+                unsigned wa = chunk_word(0), wb = chunk_word(1);
+                for (int c = 0; c < nchunk; ++c) {
+                    const unsigned w = wa;
+                    wa = wb;
+                    wb = chunk_word(c + 2);
+                    const int k = 2 * (c * kGL + gl);
+                    const bool ok0 = k < cnt, ok1 = k + 1 < cnt;
+                    // past i's count a lane reads i's own slot (inside the tile) and drops it
+                    const int j0 = ok0 ? (int)(w & 0xffffu) : islot, j1 = ok1 ? (int)(w >> 16) : islot;
+                    R dx0, dy0, dz0, dx1, dy1, dz1;
+                    disp(j0, dx0, dy0, dz0);
+                    disp(j1, dx1, dy1, dz1);
+                    if constexpr (kRel) {
+                        pp_pair_f(P, tab, ok0, dx0, dy0, dz0, fx, fy, fz, u);
+                        pp_pair_f(P, tab, ok1, dx1, dy1, dz1, fx, fy, fz, u);
+                    } else {
+                        pp_pair_masked<R>(P, tab, ok0, dx0, dy0, dz0, fx, fy, fz, u);
+                        pp_pair_masked<R>(P, tab, ok1, dx1, dy1, dz1, fx, fy, fz, u);
+                    }
+                }
+            }
+            for (int rr = 0; rr < (use_list ? 0 : 5); ++rr) {
                 int2 lr = rng[h][rr];
                 if (!have_i) lr = make_int2(0, 0);
                 const int lo = lr.x, hi = lr.y;
@@ -504,8 +643,11 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         const int cnt = min(qmine, kGL);
                         if (gl < cnt) {
                             const int jj = q[gl];
+                            if (build) list_put(jj, (grp ? lc1 : lc0) + gl);
                             pair_at(jj);
                         }
+                        lc0 += min(qn0, kGL);
+                        lc1 += min(qn1, kGL);
                         const int rest = qmine - cnt;
                         const int moved = gl < rest ? q[kGL + gl] : 0;
                         if (gl < rest) q[gl] = moved;
@@ -541,7 +683,13 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             const int qmine = grp ? qn1 : qn0;
             if (gl < qmine) {
                 const int jj = q[gl];
+                if (build) list_put(jj, (grp ? lc1 : lc0) + gl);
                 pair_at(jj);
+            }
+            if (build && gl == 0 && have_i) {
+                const int total = (grp ? lc1 : lc0) + qmine;
+                P.vl_cnt[gi] = min(total, P.vl_cap);
+                if (total > P.vl_cap) atomicOr(&P.vl_flags[1], 1);
             }
             double sfx, sfy, sfz, su;
             if constexpr (sizeof(R) == 4) {
@@ -564,7 +712,9 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 }
             }
             if (gl == 0 && have_i) {
-                double *a = P.acc + 3 * (size_t)gi;
+                // list mode: frozen slot gi's force to its current sorted slot
+                const int so = use_list ? P.inv_perm[P.vl_perm[gi]] : gi;
+                double *a = P.acc + 3 * (size_t)so;
                 if (pass == 0) {
                     a[0] = sfx;
                     a[1] = sfy;
@@ -594,18 +744,21 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
 // The FP64 12-wave kernel asks for 6 waves a SIMD (80 VGPR, a 48-byte spill
 // against 95 VGPR): two of its workgroups share a CU where one 16-wave
 // workgroup did (rc 7 at unit density: PP 3.10 -> 2.49 ms).
-template <class R, int NW>
-__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 && NW == 12 ? 6 : 1)))
+// LISTS: the kernel of a context with pair lists (build or list mode, read
+// from vl_flags[3] on the device); without, the list code is compiled out and
+// the register count stays that of the sweep alone.
+template <class R, int NW, bool LISTS>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(NW == 12 ? 6 : 1)))
 void k_pp(PPArgs<R> P) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
-    pp_body<R, NW>(P, blk_plain(), ldsraw);
+    pp_body<R, NW, LISTS>(P, blk_plain(), ldsraw);
 }
 template <class R, int NW>
 __global__ __launch_bounds__(NW * 64) void k_pp_seg(const PPArgs<R> *__restrict__ tab, SegGrid S) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
     Blk b;
     const int r = seg_locate(S, b);
-    pp_body<R, NW>(tab[r], b, ldsraw);
+    pp_body<R, NW, false>(tab[r], b, ldsraw);
 }
 
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
@@ -617,48 +770,70 @@ size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
 }
 
 template <class R>
-static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb) {
+static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, bool lists) {
     PPArgs<R> P;
     P.g = c->g;
-    P.spos = c->spos;
-    P.cell_start = c->cell_start;
+    lists = lists && c->vl_skin > 0.0;
+    P.spos = lists ? c->vl_pos : c->spos;
+    P.cell_start = lists ? c->vl_cs : c->cell_start;
+    P.vl_flags = lists ? c->vl_flags : nullptr;
+    P.vl_list = c->vl_list;
+    P.vl_cnt = c->vl_cnt;
+    P.vl_perm = c->vl_perm;
+    P.inv_perm = c->inv_perm;
+    P.vl_cap = c->vl_cap;
+    const double rl = (c->g.rc + c->vl_skin) * (1.0 + 1e-6);
+    P.rcl2 = (R)(rl * rl);
     P.cheb = cheb;
     P.acc = c->acc_pp_s;
     P.part = c->part_pp;
     P.stats = c->stats;
     P.tile_max = c->tile_max;
     P.npiece = c->npiece;
+    P.ntab = pp_table_pieces(c);
     P.piece_w = (R)(c->g.rc / c->npiece);
     P.inv_piece_w = (R)(c->npiece / c->g.rc);
+    P.rtab = sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->chebf : nullptr;
+    P.s_base = c->sp.base;
+    P.s_shift = c->sp.shift;
+    P.s_lo = c->sp.s_lo;
+    P.s_tscale = c->sp.tscale;
+    P.s_toff = c->sp.toff;
     const double rcm = c->g.rc * (1.0 + 1e-12);
     P.rc2 = (R)(c->g.rc * c->g.rc);
     P.rc2m = sizeof(R) == 4 ? P.rc2 : (R)(rcm * rcm);
     return P;
 }
 
-template <class R>
-static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_t s) {
-    const PPArgs<R> P = pp_args<R>(c, cheb);
+template <class R, bool LISTS>
+static int launch_pp_k(mdf_p3m_ctx *c, const PPArgs<R> &P, hipStream_t s) {
     if (c->pp_waves == 16) {
-        MDF_TRY(allow_lds(k_pp<R, 16>, c->pp_lds));
-        hipLaunchKernelGGL((k_pp<R, 16>), dim3(c->nblock), dim3(16 * 64), c->pp_lds, s, P);
+        MDF_TRY(allow_lds(k_pp<R, 16, LISTS>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 16, LISTS>), dim3(c->nblock), dim3(16 * 64), c->pp_lds, s, P);
     } else if (c->pp_waves == 12) {
-        MDF_TRY(allow_lds(k_pp<R, 12>, c->pp_lds));
-        hipLaunchKernelGGL((k_pp<R, 12>), dim3(c->nblock), dim3(12 * 64), c->pp_lds, s, P);
+        MDF_TRY(allow_lds(k_pp<R, 12, LISTS>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 12, LISTS>), dim3(c->nblock), dim3(12 * 64), c->pp_lds, s, P);
     } else if (c->pp_waves == 8) {
-        MDF_TRY(allow_lds(k_pp<R, 8>, c->pp_lds));
-        hipLaunchKernelGGL((k_pp<R, 8>), dim3(c->nblock), dim3(8 * 64), c->pp_lds, s, P);
+        MDF_TRY(allow_lds(k_pp<R, 8, LISTS>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 8, LISTS>), dim3(c->nblock), dim3(8 * 64), c->pp_lds, s, P);
     } else {
-        MDF_TRY(allow_lds(k_pp<R, 4>, c->pp_lds));
-        hipLaunchKernelGGL((k_pp<R, 4>), dim3(c->nblock), dim3(4 * 64), c->pp_lds, s, P);
+        MDF_TRY(allow_lds(k_pp<R, 4, LISTS>, c->pp_lds));
+        hipLaunchKernelGGL((k_pp<R, 4, LISTS>), dim3(c->nblock), dim3(4 * 64), c->pp_lds, s, P);
     }
     MDF_LAUNCHED();
     return MDF_OK;
 }
 
+template <class R>
+static int launch_pp(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, hipStream_t s) {
+    const PPArgs<R> P = pp_args<R>(c, cheb, true);
+    if (P.vl_flags) return launch_pp_k<R, true>(c, P, s);
+    return launch_pp_k<R, false>(c, P, s);
+}
+
 int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s) {
     (void)n;
-    if (c->prm.precision == MDF_PREC_MIXED) return launch_pp<float>(c, c->chebf, s);
+    if (c->prm.precision == MDF_PREC_MIXED) return launch_pp<float>(c, c->stab, s);
     return launch_pp<double>(c, c->cheb, s);
 }
 
@@ -669,14 +844,15 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
         set_error("replica batch: mixed precisions");
         return MDF_EINVAL;
     }
-    *(PPArgs<R> *)args = pp_args<R>(c, sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->chebf
-                                                    : (const typename Vec2<R>::t *)c->cheb);
+    // replica batches step without pair lists
+    *(PPArgs<R> *)args = pp_args<R>(c, sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->stab
+                                                    : (const typename Vec2<R>::t *)c->cheb, false);
     dims = make_int4(c->nblock, 1, 1, 0);
     // segmented launches run 16 waves for tiles above 64 KB, else 8 (FP32) or
     // 4 (FP64), whatever the replica's own launch uses; the LDS is sized for that
     const bool mixed = sizeof(R) == 4;
-    const int nw = pp_lds_bytes(c->tile_max, c->npiece, mixed, 4) > 64 * 1024 ? 16 : (mixed ? 8 : 4);
-    lds = pp_lds_bytes(c->tile_max, c->npiece, mixed, nw);
+    const int nw = pp_lds_bytes(c->tile_max, pp_table_pieces(c), mixed, 4) > 64 * 1024 ? 16 : (mixed ? 8 : 4);
+    lds = pp_lds_bytes(c->tile_max, pp_table_pieces(c), mixed, nw);
     return MDF_OK;
 }
 // A launch's LDS is the largest of its replicas' (each computed for the

@@ -68,6 +68,8 @@ SIGNATURES = {
     "mdf_density_modes": [_vp, _vp, _i, _vp, _i, _vp, _vp],
     "mdf_md_run": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _i, _d, _vp, _vp, _vp],
     "mdf_p3m_stats": [_vp, ct.POINTER(ct.c_int)],
+    "mdf_p3m_set_skin": [_vp, _d],
+    "mdf_p3m_list_stats": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_config": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_profile": [_vp, _i, _vp, _i],
     "mdf_batch_create": [_i, ct.POINTER(P3MParams), _i, ct.POINTER(_vp)],

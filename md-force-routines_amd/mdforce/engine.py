@@ -112,6 +112,19 @@ class P3M:
         check(self._lib.mdf_p3m_config(self._ctx, out), "mdf_p3m_config")
         return {"pp_waves": out[0], "tile_max": out[1], "npiece": out[2], "pp_lds_bytes": out[3]}
 
+    def set_skin(self, skin):
+        """Verlet pair lists of skin ``skin`` for the PP stage (0 = off): the
+        cells become L / floor(L / (rc + skin)) wide and candidate tests run
+        only when some particle has moved more than skin / 2 since the last
+        build.  Forces are unchanged up to summation order."""
+        check(self._lib.mdf_p3m_set_skin(self._ctx, float(skin)), "mdf_p3m_set_skin")
+        self.skin = float(skin)
+
+    def list_stats(self):
+        out = (ct.c_int * 4)()
+        check(self._lib.mdf_p3m_list_stats(self._ctx, out), "mdf_p3m_list_stats")
+        return {"builds": out[0], "list_steps": out[1], "list_cap": out[2], "cells_x": out[3]}
+
     STAGES = ("bin", "pp", "assign", "fft", "gather", "drift", "modes", "finalize")
 
     def profile_start(self, events=65536):

@@ -1,0 +1,132 @@
+"""Verlet pair lists (mdf_p3m_set_skin) against the list-free engine and the
+oracle.  With a skin the PP stage tests candidates only when some ion has
+moved more than skin / 2 since the last build; every other step evaluates the
+listed pairs.  The forces must be those of the list-free path up to summation
+order: relative RMS <= 1e-12 in FP64 (the oracle's bar) and <= 2e-6 in the
+FP32-mixed form (two FP32 summation orders)."""
+import numpy as np
+import pytest
+
+from oracle import p3m_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def rel_rms(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.sqrt(np.mean(np.sum((a - b) ** 2, -1)) / np.mean(np.sum(b ** 2, -1))))
+
+
+@pytest.fixture(scope="module")
+def P3M():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test run without a GPU")
+    from mdforce.engine import P3M as cls
+    return cls
+
+
+@pytest.fixture(scope="module")
+def system():
+    # a jittered simple-cubic lattice (no close pairs: the lists live for many
+    # steps, as in the liquid)
+    n1 = 27
+    N = n1 ** 3
+    L = O.box_length(N)
+    kappa, alpha, rc, M = 1.0, 0.7, 4.5, 32
+    P = O.P3MParams(N, L, kappa, alpha, rc, M)
+    g = (np.arange(n1) + 0.5) * (L / n1)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = pos + np.random.RandomState(5).uniform(-0.25, 0.25, pos.shape) * (L / n1)
+    _, vel = O.initial_split(5, 6, N, P.Lv, 0.01)
+    # a common drift carries every ion across the periodic boundary within a
+    # list's lifetime, which the frozen order has to follow
+    vel = vel + np.array([0.6, -0.45, 0.3])
+    return dict(P=P, pos=pos, vel=vel, N=N, L=L, M=M, kappa=kappa, alpha=alpha, rc=rc)
+
+
+@pytest.mark.parametrize("precision,skin", [("fp64", 0.5), ("mixed", 0.5), ("fp64", 0.25)])
+def test_md_run_with_lists_matches_list_free_forces(P3M, system, precision, skin):
+    S = system
+    P, N = S["P"], S["N"]
+    e = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision=precision)
+    e.set_green(dev(P.G_k))
+    e.set_skin(skin)
+    ref = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision=precision)
+    ref.set_green(dev(P.G_k))
+    pos, vel = dev(S["pos"]), dev(S["vel"])
+    acc, _ = e.force(pos)
+    acc = acc.clone()
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    total_list_steps = 0
+    for chunk in range(3):
+        en, _ = e.md_run(pos, vel, acc, 7, 0.02)
+        torch.cuda.synchronize()
+        st = e.list_stats()
+        # acc holds the force of the final positions, from the lists unless the
+        # last step built; either way it must equal the list-free force
+        a_ref, en_ref = ref.force(pos)
+        torch.cuda.synchronize()
+        err = rel_rms(acc.cpu().numpy(), a_ref.cpu().numpy())
+        assert err <= tol, (chunk, err, st)
+        U = float(en[-1, 2])
+        assert abs(U - float(en_ref[0])) <= tol * abs(float(en_ref[0])), (chunk, U, float(en_ref[0]))
+        total_list_steps = st["list_steps"]
+    st = e.list_stats()
+    assert st["builds"] >= 3 and total_list_steps >= 6, st
+    # positions left [0, L) and came back by the drift's wrap; the oracle agrees on the end state
+    if precision == "fp64":
+        U_o, a_o, _ = P.force(pos.cpu().numpy())
+        assert rel_rms(acc.cpu().numpy(), a_o) <= 1e-12
+    e.close()
+    ref.close()
+
+
+def test_list_build_on_large_displacement_and_skin_off(P3M, system):
+    """A caller that moves ions between calls gets a build at the first step;
+    skin 0 turns the lists off and keeps the context usable."""
+    S = system
+    P, N = S["P"], S["N"]
+    e = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision="fp64")
+    e.set_green(dev(P.G_k))
+    e.set_skin(0.4)
+    pos, vel = dev(S["pos"]), dev(S["vel"]) * 0.0
+    acc, _ = e.force(pos)
+    acc = acc.clone()
+    e.md_run(pos, vel, acc, 3, 0.01)
+    rng = np.random.RandomState(9)
+    moved = np.mod(S["pos"] + rng.uniform(-2.0, 2.0, S["pos"].shape), S["L"])  # in [0, L), as the oracle needs
+    pos.copy_(dev(moved))
+    U_o, a_o, _ = P.force(pos.cpu().numpy())
+    a1, _ = e.force(pos)  # force() always builds
+    torch.cuda.synchronize()
+    assert rel_rms(a1.cpu().numpy(), a_o) <= 1e-12
+    acc = a1.clone()
+    e.md_run(pos, vel, acc, 2, 1e-6)  # first step builds; the second uses the lists
+    torch.cuda.synchronize()
+    U_o, a_o, _ = P.force(pos.cpu().numpy())
+    assert rel_rms(acc.cpu().numpy(), a_o) <= 1e-12
+    e.set_skin(0.0)
+    assert e.config()["pp_waves"] > 0
+    a2, _ = e.force(pos)
+    torch.cuda.synchronize()
+    assert rel_rms(a2.cpu().numpy(), a_o) <= 1e-12
+    e.close()
+
+
+def test_skin_too_large_for_the_box_is_refused(P3M, system):
+    from mdforce._lib import MdforceError
+    S = system
+    e = P3M(S["N"], S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision="mixed")
+    with pytest.raises(MdforceError, match="L/\\(rc \\+ skin\\) >= 3"):
+        e.set_skin(S["L"] / 3.0 - S["rc"] + 0.5)
+    e.build_green()  # the context is unchanged and still works
+    a, _ = e.force(dev(S["pos"]))
+    torch.cuda.synchronize()
+    assert torch.isfinite(a).all()
+    e.close()
