@@ -64,7 +64,14 @@ def parse():
     ap.add_argument("--no-err", action="store_true", help="skip the force-error measurement")
     ap.add_argument("--no-liquid", action="store_true",
                     help="measure the force error on the uniform start only (no equilibrated state)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="NVE steps of the all-core CPU baseline (the C restatement on all host threads)")
+    ap.add_argument("--cpu-steps1", type=int, default=1,
+                    help="NVE steps of the single-thread CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-core baseline (0 = all)")
+    ap.add_argument("--skin", type=float, default=0.05,
+                    help="Verlet pair-list skin of the PP stage (0 = off); the cells of rc + skin set the "
+                         "skin actually used (the largest they allow)")
     ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "drift", "sqw"), default="n1e6",
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
@@ -75,8 +82,9 @@ def parse():
     ap.add_argument("--uniform-steps", type=int, default=1000)
     ap.add_argument("--start", choices=("both", "uniform", "equilibrated"), default="both",
                     help="drift workload: start(s) of the trajectories")
-    ap.add_argument("--autotune", action="store_true",
-                    help="alpha and rc from the autotuner (FP32-mixed, budget 1e-5, the given mesh)")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="alpha and rc from the autotuner (FP32-mixed, budget 1e-5, the given mesh, tuned on the "
+                         "equilibrated liquid); --no-autotune uses --alpha/--rc")
     ap.add_argument("--graph-steps", type=int, default=50, help="0 = no graph (eager launches)")
     ap.add_argument("--streams", type=int, default=16, help="worker streams of the replica graph")
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both",
@@ -106,6 +114,23 @@ def pmc_traffic(stage, form, N, M, rc, alpha):
     except OSError:
         return None
     return None
+
+
+def host_cpu():
+    """Model name and thread counts of the host CPU."""
+    model = platform.processor() or "cpu"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return {"model": model, "nproc": avail, "cpu_count": os.cpu_count()}
 
 
 def stage_work(stage, N, M, rc):
@@ -552,8 +577,10 @@ def main():
     dq = 2.0 * math.pi / L
     # Equilibrated liquid of the same system: the force budget is relative to
     # the mean force, which is ~20x smaller there than on the uniform start.
-    pos_liq = None
-    if not (a.no_err and not a.autotune) and not a.no_liquid:
+    # The timed loop starts from this liquid too, so that the step time, the
+    # force error and the energy drift all refer to one state.
+    pos_liq = vel_liq = None
+    if not a.no_liquid:
         from mdforce import mlp_train
         t_eq = time.perf_counter()
         pos_liq = torch.from_numpy(pos0).to(dev)
@@ -562,7 +589,6 @@ def main():
         eq.build_green()
         mlp_train.equilibrate(eq, pos_liq, vel_liq, T, a.eq_steps, a.dt, vcap=4.0, soft_steps=100)
         eq.close()
-        del vel_liq
         torch.cuda.synchronize()
         print(f"[bench] equilibrated {a.eq_steps} + 100 steps in {time.perf_counter() - t_eq:.1f} s",
               file=sys.stderr, flush=True)
@@ -617,13 +643,18 @@ def main():
             out["reference"] = f"FP64 P3M {ref_set[0]}^3, alpha {ref_set[1]:.3f}, rc {ref_set[2]:.2f}"
         return out
 
+    start_pos = pos_liq if pos_liq is not None else torch.from_numpy(pos0).to(dev)
+    start_vel = vel_liq if vel_liq is not None else torch.from_numpy(vel0).to(dev)
+
     def run_form(precision):
         eng = P3M(N, M, L, a.kappa, a.alpha, a.rc, precision=precision, device=dev)
-        pp_cfg = eng.config()
         G_full, _ = eng.build_green(6, 3, want_full=True)
-        pos = torch.from_numpy(pos0).to(dev)
-        vel = torch.from_numpy(vel0).to(dev)
         err = force_error(eng)
+        if a.skin > 0:
+            eng.set_skin(a.skin)
+        pp_cfg = eng.config()
+        pos = start_pos.clone()
+        vel = start_vel.clone()
         acc, _ = eng.force(pos)
         acc = acc.clone()
         torch.cuda.synchronize()
@@ -633,6 +664,7 @@ def main():
         energies = torch.empty((a.steps, 4), dtype=torch.float64, device=dev)
         nqt = torch.empty((a.steps, a.nq, 3), dtype=torch.complex128, device=dev)
         torch.cuda.synchronize()
+        ls0 = eng.list_stats() if a.skin > 0 else None
         eng.profile_start(events=16 * (a.steps + 4))
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
@@ -646,11 +678,18 @@ def main():
         gpu_ms = start.elapsed_time(stop)
         stages = eng.profile_read()
         eng.profile_stop()
+        lists = None
+        if a.skin > 0:
+            ls1 = eng.list_stats()
+            lists = {"skin_requested": a.skin, "cells_x": ls1["cells_x"],
+                     "skin_effective": L / ls1["cells_x"] - a.rc, "list_cap": ls1["list_cap"],
+                     "builds_timed": ls1["builds"] - ls0["builds"],
+                     "list_steps_timed": ls1["list_steps"] - ls0["list_steps"]}
         en_h = energies.cpu().numpy()
         drift = float(np.max(np.abs(en_h[:, 0] - en_h[0, 0])) / (N * T))
         eng.close()
         return {"gpu_ms": gpu_ms, "wall": wall, "stages": stages, "err": err, "drift": drift, "G_full": G_full,
-                "pp_config": pp_cfg,
+                "pp_config": pp_cfg, "lists": lists,
                 "acc0": acc_init, "value": N * a.steps / (gpu_ms / 1e3)}
 
     forms = {"both": ("fp64", "mixed"), "chosen": ("mixed",), "plain": ("fp64",)}[a.form]
@@ -667,13 +706,15 @@ def main():
     avg_s = tot_ms / max(cnt, 1) / 1e3
     by, fl = stage_work(dom, N, M, a.rc)
     if fl > 0:
+        # the pair terms run on the vector ALUs (no matrix instructions)
         peak = PEAK_FP32 if main_form == "mixed" else PEAK_FP64
-        roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+        roof = {"bound": "valu", "achieved": fl / avg_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
     else:
         roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = pmc_traffic(dom, main_form, N, M, a.rc, a.alpha)
-    pp_name = f"k_pp<{'float' if main_form == 'mixed' else 'double'}, {R['pp_config']['pp_waves']}>"
+    pp_name = (f"k_pp<{'float' if main_form == 'mixed' else 'double'}, {R['pp_config']['pp_waves']}, "
+               f"{'true' if R['lists'] else 'false'}>")
     roof["kernel"] = {"pp": pp_name, "fft": "fft passes", "gather": "k_gather", "assign": "k_assign",
                       "bin": "bin kernels", "drift": "k_kick_drift", "modes": "k_modes_grid"}.get(dom, dom)
     roof["avg_launch_ms"] = avg_s * 1e3
@@ -686,7 +727,7 @@ def main():
         form_cfg["chosen" if f == "mixed" else "plain"] = {
             "particle_steps_per_s": r["value"], "ms_per_step": r["gpu_ms"] / a.steps, "force_error": r["err"],
             "energy_drift_per_NT": r["drift"], "stage_ms": stage_ms(r["stages"]), "wall_s": r["wall"],
-            "pp_config": r["pp_config"]}
+            "pp_config": r["pp_config"], "pair_lists": r["lists"]}
     if len(res) == 2:
         form_cfg["chosen_over_plain"] = res["mixed"]["value"] / res["fp64"]["value"]
     G_full = R["G_full"]
@@ -698,20 +739,33 @@ def main():
         from oracle import c_oracle
         c_oracle.build(ref=False)
         Gh = G_full.cpu().numpy()
-        p_h, v_h = pos0.copy(), vel0.copy()
+        p0h, v0h = start_pos.cpu().numpy(), start_vel.cpu().numpy()
         if acc0 is None:  # initial FP64 force, equal to the C restatement's to 1e-12
             e64 = P3M(N, M, L, a.kappa, a.alpha, a.rc, device=dev)
             e64.build_green(6, 3)
-            acc0 = e64.force(torch.from_numpy(pos0).to(dev))[0].cpu().numpy()
+            acc0 = e64.force(start_pos)[0].cpu().numpy()
             e64.close()
-        a_h = acc0.copy()
         qh = q.cpu().numpy()
-        t0 = time.perf_counter()
-        c_oracle.md_steps(p_h, v_h, a_h, np.ones(N), a.cpu_steps, a.dt, L, a.kappa, a.alpha, a.rc, Gh, M, q=qh)
-        tc = time.perf_counter() - t0
-        cpu = {"value": N * a.cpu_steps / tc, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{a.cpu_steps} NVE step(s) of the same N={N}, {M}^3 workload, single-threaded C "
-                         f"restatement (oracle/c/p3m_oracle.c), {tc:.1f} s, host {platform.processor() or 'cpu'}"}
+        host = host_cpu()
+        state = "equilibrated liquid" if pos_liq is not None else "uniform start"
+
+        def cpu_run(steps, threads):
+            p_h, v_h, a_h = p0h.copy(), v0h.copy(), acc0.copy()
+            t0 = time.perf_counter()
+            c_oracle.md_steps(p_h, v_h, a_h, np.ones(N), steps, a.dt, L, a.kappa, a.alpha, a.rc, Gh, M, q=qh,
+                              threads=threads)
+            return time.perf_counter() - t0
+
+        nthr = c_oracle.lib_omp(a.cpu_threads)[1]
+        tc = cpu_run(a.cpu_steps, a.cpu_threads)
+        cpu = {"value": N * a.cpu_steps / tc, "unit": "particle-steps/s", "cores": nthr, "kind": "port",
+               "sample": f"{a.cpu_steps} NVE steps of the same N={N}, {M}^3 workload from the {state}, the C "
+                         f"restatement of the reference algorithm (oracle/c/p3m_oracle.c, OpenMP, -O3) on "
+                         f"{nthr} threads, {tc:.1f} s; host {host['model']}, nproc {host['nproc']}"}
+        if a.cpu_steps1 > 0:
+            t1 = cpu_run(a.cpu_steps1, None)
+            cpu["single_thread"] = {"value": N * a.cpu_steps1 / t1, "cores": 1,
+                                    "sample": f"{a.cpu_steps1} NVE step(s), serial build (-O2), {t1:.1f} s"}
 
     out = {
         "metric": METRIC,

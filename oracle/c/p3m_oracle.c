@@ -17,10 +17,50 @@
  * Energy sums (U_pp over pairs, U_pm over modes) use Neumaier compensated
  * summation: a plain running sum over ~1e8 pair energies drifts by ~1e-12
  * relative, which is the size of the parity bar this checker enforces.
+ *
+ * Built with OpenMP (c/liboracle_p3m_omp.so, the all-core CPU baseline) the
+ * same loops run on all threads: cells, particles, mesh lines and modes are
+ * split over threads; the Newton-3 pair updates and the charge deposits go to
+ * per-thread arrays summed afterwards in thread order.  The serial build
+ * (c/liboracle_p3m.so) is the checker.
  */
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the OpenMP build (n > 0 sets them); 1 in the serial build. */
+int orc_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+static int nthreads_now(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static int thread_id(void)
+{
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -81,10 +121,19 @@ int orc_pp(const double *pos, int n, const double *Lv, double kappa, double G, d
         ls[i] = head[c];
         head[c] = i;
     }
-    csum U = {0.0, 0.0};
-    for (int cx = 0; cx < nd[0]; ++cx)
-    for (int cy = 0; cy < nd[1]; ++cy)
-    for (int cz = 0; cz < nd[2]; ++cz) {
+    const int nt = nthreads_now();
+    double *tacc = acc;
+    csum *tU = calloc((size_t)nt, sizeof(csum));
+    if (nt > 1) tacc = calloc((size_t)nt * 3 * (size_t)n, sizeof(double));
+    if (!tU || !tacc) { free(head); free(ls); free(tU); if (tacc != acc) free(tacc); return -2; }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int cell = 0; cell < ncell; ++cell) {
+        const int cx = cell % nd[0], cy = (cell / nd[0]) % nd[1], cz = cell / (nd[0] * nd[1]);
+        const int tid = thread_id();
+        double *acc_t = nt > 1 ? tacc + (size_t)tid * 3 * n : acc;
+        csum *U = &tU[tid];
         int c = cx + cy * nd[0] + cz * nd[0] * nd[1];
         for (int czN = cz - 1; czN <= cz + 1; ++czN)
         for (int cyN = cy - 1; cyN <= cy + 1; ++cyN)
@@ -107,22 +156,39 @@ int orc_pp(const double *pos, int n, const double *Lv, double kappa, double G, d
                         if (r < rc) {
                             double phi, fr;
                             pair_terms(r, kappa, G, &phi, &fr);
-                            csum_add(&U, phi);
-                            acc[3 * i + 0] += fr * dx / r;
-                            acc[3 * i + 1] += fr * dy / r;
-                            acc[3 * i + 2] += fr * dz / r;
-                            acc[3 * j + 0] -= fr * dx / r;
-                            acc[3 * j + 1] -= fr * dy / r;
-                            acc[3 * j + 2] -= fr * dz / r;
+                            csum_add(U, phi);
+                            acc_t[3 * i + 0] += fr * dx / r;
+                            acc_t[3 * i + 1] += fr * dy / r;
+                            acc_t[3 * i + 2] += fr * dz / r;
+                            acc_t[3 * j + 0] -= fr * dx / r;
+                            acc_t[3 * j + 1] -= fr * dy / r;
+                            acc_t[3 * j + 2] -= fr * dz / r;
                         }
                     }
                 }
             }
         }
     }
+    if (nt > 1) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (long k = 0; k < 3L * n; ++k) {
+            double a = 0.0;
+            for (int t = 0; t < nt; ++t) a += tacc[(size_t)t * 3 * n + k];
+            acc[k] = a;
+        }
+        free(tacc);
+    }
+    csum Ut = {0.0, 0.0};
+    for (int t = 0; t < nt; ++t) {
+        csum_add(&Ut, tU[t].s);
+        csum_add(&Ut, tU[t].c);
+    }
+    free(tU);
     free(head);
     free(ls);
-    *U_out = csum_get(&U);
+    *U_out = csum_get(&Ut);
     return 0;
 }
 
@@ -157,8 +223,19 @@ int orc_charge_assign(const double *pos, const double *Z, int n, int Mx, int My,
 {
     int M[3] = {Mx, My, Mz};
     double h[3] = {hx, hy, hz};
-    memset(rho, 0, (size_t)Mx * My * Mz * sizeof(double));
+    const size_t mv = (size_t)Mx * My * Mz;
+    memset(rho, 0, mv * sizeof(double));
+    const int nt = nthreads_now();
+    double *trho = rho;
+    if (nt > 1) {
+        trho = calloc((size_t)nt * mv, sizeof(double));
+        if (!trho) return -2;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int ip = 0; ip < n; ++ip) {
+        double *rho_t = nt > 1 ? trho + (size_t)thread_id() * mv : rho;
         int b[3];
         double w[3][6];
         stencil(pos + 3 * ip, M, h, b, w);
@@ -168,10 +245,21 @@ int orc_charge_assign(const double *pos, const double *Z, int n, int Mx, int My,
                 int ri = wrap1(b[1] + i, My);
                 for (int j = 0; j < 6; ++j) {
                     int rj = wrap1(b[0] + j, Mx);
-                    rho[((size_t)rg * My + ri) * Mx + rj] += Z[ip] * w[2][g] * w[1][i] * w[0][j];
+                    rho_t[((size_t)rg * My + ri) * Mx + rj] += Z[ip] * w[2][g] * w[1][i] * w[0][j];
                 }
             }
         }
+    }
+    if (nt > 1) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (long k = 0; k < (long)mv; ++k) {
+            double a = 0.0;
+            for (int t = 0; t < nt; ++t) a += trho[(size_t)t * mv + k];
+            rho[k] = a;
+        }
+        free(trho);
     }
     return 0;
 }
@@ -182,6 +270,9 @@ int orc_field_gather(const double *Ex, const double *Ey, const double *Ez, const
 {
     int M[3] = {Mx, My, Mz};
     double h[3] = {hx, hy, hz};
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int ip = 0; ip < n; ++ip) {
         int b[3];
         double w[3][6];
@@ -256,26 +347,37 @@ static int fft3d(cplx *c, int Mx, int My, int Mz, int sign)
 {
     int mm = Mx > My ? Mx : My;
     mm = mm > Mz ? mm : Mz;
-    cplx *line = malloc((size_t)mm * sizeof(cplx));
-    cplx *tmp = malloc((size_t)mm * sizeof(cplx));
-    if (!line || !tmp) { free(line); free(tmp); return -2; }
-    for (int z = 0; z < Mz; ++z)
-        for (int y = 0; y < My; ++y)
-            fft1d(c + ((size_t)z * My + y) * Mx, Mx, sign, tmp);
-    for (int z = 0; z < Mz; ++z)
-        for (int x = 0; x < Mx; ++x) {
-            for (int y = 0; y < My; ++y) line[y] = c[((size_t)z * My + y) * Mx + x];
-            fft1d(line, My, sign, tmp);
-            for (int y = 0; y < My; ++y) c[((size_t)z * My + y) * Mx + x] = line[y];
-        }
-    for (int y = 0; y < My; ++y)
-        for (int x = 0; x < Mx; ++x) {
-            for (int z = 0; z < Mz; ++z) line[z] = c[((size_t)z * My + y) * Mx + x];
-            fft1d(line, Mz, sign, tmp);
-            for (int z = 0; z < Mz; ++z) c[((size_t)z * My + y) * Mx + x] = line[z];
-        }
-    free(line);
-    free(tmp);
+    const int nt = nthreads_now();
+    cplx *lines = malloc((size_t)nt * 2 * mm * sizeof(cplx));  /* per thread: line, tmp */
+    if (!lines) return -2;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int zy = 0; zy < Mz * My; ++zy) {
+        cplx *tmp = lines + (size_t)thread_id() * 2 * mm + mm;
+        fft1d(c + (size_t)zy * Mx, Mx, sign, tmp);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int zx = 0; zx < Mz * Mx; ++zx) {
+        const int z = zx / Mx, x = zx % Mx;
+        cplx *line = lines + (size_t)thread_id() * 2 * mm, *tmp = line + mm;
+        for (int y = 0; y < My; ++y) line[y] = c[((size_t)z * My + y) * Mx + x];
+        fft1d(line, My, sign, tmp);
+        for (int y = 0; y < My; ++y) c[((size_t)z * My + y) * Mx + x] = line[y];
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int yx = 0; yx < My * Mx; ++yx) {
+        const int y = yx / Mx, x = yx % Mx;
+        cplx *line = lines + (size_t)thread_id() * 2 * mm, *tmp = line + mm;
+        for (int z = 0; z < Mz; ++z) line[z] = c[((size_t)z * My + y) * Mx + x];
+        fft1d(line, Mz, sign, tmp);
+        for (int z = 0; z < Mz; ++z) c[((size_t)z * My + y) * Mx + x] = line[z];
+    }
+    free(lines);
     return 0;
 }
 
@@ -354,16 +456,26 @@ int orc_pm(const double *pos, const double *Z, int n, const double *G_k, int Mx,
     /* natural index n <-> shifted index (n + M/2) % M <-> k = 2 pi (s - M/2)/L */
     csum U = {0.0, 0.0};
     for (int a = 0; a < 3; ++a) {
-        for (int z = 0; z < Mz; ++z)
-        for (int y = 0; y < My; ++y)
+        if (a == 0) {  /* U_f, in the serial order */
+            for (size_t i = 0; i < mv; ++i) {
+                const int x = (int)(i % Mx), y = (int)((i / Mx) % My), z = (int)(i / ((size_t)Mx * My));
+                const int sx = (x + Mx / 2) % Mx, sy = (y + My / 2) % My, sz = (z + Mz / 2) % Mz;
+                const double g = G_k[((size_t)sz * My + sy) * Mx + sx];
+                csum_add(&U, (rk[i].re * rk[i].re + rk[i].im * rk[i].im) * g);
+            }
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int zy = 0; zy < Mz * My; ++zy)
         for (int x = 0; x < Mx; ++x) {
+            const int z = zy / My, y = zy % My;
             int nat[3] = {x, y, z};
             int sh[3];
             for (int d = 0; d < 3; ++d) sh[d] = (nat[d] + Ms[d] / 2) % Ms[d];
             size_t i = ((size_t)z * My + y) * Mx + x;
             double g = G_k[((size_t)sh[2] * My + sh[1]) * Mx + sh[0]];
             double pr = g * rk[i].re, pi = g * rk[i].im;
-            if (a == 0) csum_add(&U, (rk[i].re * rk[i].re + rk[i].im * rk[i].im) * g);
             double k = 2 * M_PI * (sh[a] - Ms[a] / 2) / Ls[a];
             /* E_k = -i k phi_k */
             ek[i].re = k * pi;
@@ -414,6 +526,9 @@ int orc_md_steps(double *pos, double *vel, double *acc, const double *Z, int n, 
     double *apm = malloc((size_t)n * 3 * sizeof(double));
     if (!app || !apm) { free(app); free(apm); return -2; }
     for (int s = 0; s < nsteps; ++s) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
         for (int i = 0; i < 3 * n; ++i) {
             vel[i] += 0.5 * acc[i] * dt;
             pos[i] += vel[i] * dt;
@@ -437,6 +552,9 @@ int orc_md_steps(double *pos, double *vel, double *acc, const double *Z, int n, 
             energies[4 * s + 3] = K / kf / n;
         }
         if (nqt) {
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static)
+#endif
             for (int iq = 0; iq < nq; ++iq)
                 for (int a = 0; a < 3; ++a) {
                     double sr = 0, si = 0;

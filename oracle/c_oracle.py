@@ -34,6 +34,28 @@ def build(ref: bool | None = None):
 
 _lib = None
 _ref = None
+_omp = None
+ORACLE_OMP_SO = os.path.join(HERE, "c", "liboracle_p3m_omp.so")
+
+
+def lib_omp(threads: int = 0):
+    """The same restatement built with OpenMP (all-core CPU baseline);
+    threads > 0 sets the thread count.  Returns (lib, threads in use)."""
+    global _omp
+    if _omp is None:
+        if not os.path.isfile(ORACLE_OMP_SO):
+            build(ref=False)
+        L = ct.CDLL(ORACLE_OMP_SO)
+        base = lib()
+        for name in ("orc_pp", "orc_charge_assign", "orc_field_gather", "orc_fft3d", "orc_gf_opt",
+                     "orc_pm", "orc_force_pot", "orc_md_steps"):
+            fn = getattr(L, name)
+            fn.argtypes = getattr(base, name).argtypes
+            fn.restype = ct.c_int
+        L.orc_threads.argtypes = [ct.c_int]
+        L.orc_threads.restype = ct.c_int
+        _omp = L
+    return _omp, _omp.orc_threads(int(threads))
 
 
 def lib():
@@ -154,14 +176,17 @@ def force_pot(pos, Z, L, kappa, G, rc, G_k, M, af=1.0 / 3.0, uf=1.0):
 
 
 def md_steps(pos, vel, acc, Z, nsteps, dt, L, kappa, G, rc, G_k, M, q=None,
-             af=1.0 / 3.0, uf=1.0, kf=1.5):
-    """Runs nsteps NVE steps in place; returns (energies (nsteps,4), nqt)."""
+             af=1.0 / 3.0, uf=1.0, kf=1.5, threads=None):
+    """Runs nsteps NVE steps in place; returns (energies (nsteps,4), nqt).
+    threads: None runs the serial checker; an int runs the OpenMP build on
+    that many threads (0: all)."""
+    L_ = lib() if threads is None else lib_omp(threads)[0]
     Lv = np.array([L, L, L])
     q = np.zeros(0) if q is None else np.ascontiguousarray(q, dtype=np.float64)
     en = np.zeros((nsteps, 4))
     nqt = np.zeros((nsteps, q.size, 3), dtype=np.complex128)
     G_k = np.ascontiguousarray(G_k, dtype=np.float64)
-    _check(lib().orc_md_steps(_ptr(pos), _ptr(vel), _ptr(acc), _ptr(np.ascontiguousarray(Z, dtype=np.float64)),
+    _check(L_.orc_md_steps(_ptr(pos), _ptr(vel), _ptr(acc), _ptr(np.ascontiguousarray(Z, dtype=np.float64)),
                               pos.shape[0], nsteps, dt, _ptr(Lv), kappa, G, rc, af, uf, kf, _ptr(G_k), M,
                               _ptr(q), q.size, _ptr(en), nqt.ctypes.data_as(_dp)), "orc_md_steps")
     return en, nqt

@@ -491,3 +491,26 @@ def test_dropin_initial_matches_oracle_and_global_stream():
         np.random.set_state(state)
     assert np.array_equal(vel, want_v)
     assert np.array_equal(pos, want_p)
+
+
+def test_openmp_restatement_matches_serial():
+    """The all-core CPU baseline (the C restatement built with OpenMP) computes
+    what the serial checker does: 2 NVE steps of a cfg1-sized system agree to
+    1e-12 in positions, forces and energies."""
+    N = 3000
+    L = O.box_length(N)
+    P = O.P3MParams(N, L, 2.0, 0.85, L / 3.1, 32)
+    pos, vel = O.initial_split(3, 4, N, P.Lv, 0.01)
+    _, acc, _ = P.force(pos)
+    runs = []
+    for threads in (None, 4):
+        p, v, a = pos.copy(), vel.copy(), acc.copy()
+        en, nqt = c_oracle.md_steps(p, v, a, np.ones(N), 2, 0.1, L, 2.0, 0.85, L / 3.1, P.G_k, 32,
+                                    q=np.array([0.3, 0.6]), threads=threads)
+        runs.append((p, v, a, en, nqt))
+    (p1, v1, a1, e1, q1), (p2, v2, a2, e2, q2) = runs
+    assert c_oracle.lib_omp(0)[1] >= 1
+    np.testing.assert_allclose(p2, p1, rtol=0, atol=1e-12)
+    assert rel_rms(a2, a1) < 1e-12
+    np.testing.assert_allclose(e2, e1, rtol=1e-12)
+    np.testing.assert_allclose(q2, q1, rtol=1e-12, atol=1e-9)
