@@ -564,9 +564,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 const int cnt = have_i ? P.vl_cnt[gi] : 0;
                 const int nchunk = (max(readlane_i(cnt, 0), readlane_i(cnt, kGL)) + 2 * kGL - 1) / (2 * kGL);
                 const unsigned *lw = (const unsigned *)lst;  // vl_cap is a multiple of 32: dword aligned
-                // the load itself is unconditional (its index clamped into i's list)
-                // so that the wait for it can be placed where its value is used
-                const int wlast = P.vl_cap / 2 - 1;
+                // the load itself is unconditional (its index clamped to i's last
+                // word, or word 0 of an empty list) so that the wait for it can be
+                // placed where its value is used
+                const int wlast = cnt > 0 ? (cnt - 1) >> 1 : 0;
                 auto chunk_word = [&](int c) -> unsigned { return lw[min(c * kGL + gl, wlast)]; };
                 unsigned wa = chunk_word(0), wb = chunk_word(1);
                 for (int c = 0; c < nchunk; ++c) {
