@@ -29,17 +29,36 @@ namespace mdf {
 // Weights of yukawa_pm_ord6.py:29-48 in Horner form, in arithmetic T.  The
 // six polynomials pair up as w[5-k](x) = w[k](-x) (the coefficients of each
 // pair differ in the sign of the odd powers), so w0, w1, w2 give all six.
+//
+// Normalisation: FP64 divides each weight by 3840 (w0, w1) or 1920 (w2) as
+// the reference does.  FP32 evaluates the numerators 3840 w, whose
+// coefficients are exact, and applies 1/3840^3 once in FP64, to each deposit
+// and to each gathered field (wscale).  The FP32 constant 1/3840 lies 5.2e-8
+// above 1/3840; taken once per axis it raised every deposit and every
+// gathered field by 1.6e-7 (sum of rho = N (1 + 1.6e-7)).
+template <class T>
+struct WNorm {
+    static constexpr T c01 = (T)(1.0 / 3840.0), c2 = (T)(1.0 / 1920.0);
+};
+template <>
+struct WNorm<float> {
+    static constexpr float c01 = 1.0f, c2 = 2.0f;
+};
+template <class T>
+__device__ __forceinline__ double wscale() {
+    return sizeof(T) == 4 ? 1.0 / (3840.0 * 3840.0 * 3840.0) : 1.0;
+}
 template <class T>
 __device__ __forceinline__ T w6_0(T x) {
-    return ((T)1 + x * ((T)-10 + x * ((T)40 + x * ((T)-80 + x * ((T)80 + x * (T)-32))))) * (T)(1.0 / 3840.0);
+    return ((T)1 + x * ((T)-10 + x * ((T)40 + x * ((T)-80 + x * ((T)80 + x * (T)-32))))) * WNorm<T>::c01;
 }
 template <class T>
 __device__ __forceinline__ T w6_1(T x) {
-    return ((T)237 + x * ((T)-750 + x * ((T)840 + x * ((T)-240 + x * ((T)-240 + x * (T)160))))) * (T)(1.0 / 3840.0);
+    return ((T)237 + x * ((T)-750 + x * ((T)840 + x * ((T)-240 + x * ((T)-240 + x * (T)160))))) * WNorm<T>::c01;
 }
 template <class T>
 __device__ __forceinline__ T w6_2(T x) {
-    return ((T)841 + x * ((T)-770 + x * ((T)-440 + x * ((T)560 + x * ((T)80 + x * (T)-160))))) * (T)(1.0 / 1920.0);
+    return ((T)841 + x * ((T)-770 + x * ((T)-440 + x * ((T)560 + x * ((T)80 + x * (T)-160))))) * WNorm<T>::c2;
 }
 template <class T>
 __device__ __forceinline__ void weights6(T x, T w[6]) {
@@ -255,14 +274,15 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
                 const T zwy = zw * wy[b];
                 for (int c = 0; c < 6; ++c) {
                     const int gx = wrap3(ix + c, g.M[0]);
-                    unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx], (double)(zwy * wx[c]));
+                    unsafeAtomicAdd(&P.rho[((size_t)gz * g.M[1] + gy) * g.M[0] + gx],
+                                    (double)(zwy * wx[c]) * wscale<T>());
                 }
             }
         }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-        const double v = brick[t];
+        const double v = brick[t] * wscale<T>();
         if (v != 0.0) {
             const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
             const int gx = wrap3(o0 + tx, g.M[0]);
@@ -421,11 +441,12 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
         ez = quad_sum(ez);
         if (!live || sub != 0) continue;
         const int i = P.perm[s];
-        const double apx = p.w * (double)ex, apy = p.w * (double)ey, apz = p.w * (double)ez;
+        const double exd = (double)ex * wscale<T>(), eyd = (double)ey * wscale<T>(), ezd = (double)ez * wscale<T>();
+        const double apx = p.w * exd, apy = p.w * eyd, apz = p.w * ezd;
         const GatherOut &o = P.o;
-        if (o.ex) o.ex[i] = (double)ex;
-        if (o.ey) o.ey[i] = (double)ey;
-        if (o.ez) o.ez[i] = (double)ez;
+        if (o.ex) o.ex[i] = exd;
+        if (o.ey) o.ey[i] = eyd;
+        if (o.ez) o.ez[i] = ezd;
         if (o.acc_pm) {
             o.acc_pm[3 * (size_t)i + 0] = apx;
             o.acc_pm[3 * (size_t)i + 1] = apy;

@@ -154,8 +154,6 @@ def evaluate(net, eng_red, pos, target, kappa, alpha_red):
     return before, after
 
 
-
-
 def judge(net, eng_red, eng_plain, pos, kappa, alpha_red):
     """Error estimate of the corrected reduced force: RMS distance to the plain
     P3M force over the mean plain force magnitude (plain P3M is the judge)."""
@@ -171,3 +169,42 @@ def choose_form(net, eng_red, eng_plain, pos, kappa, alpha_red, budget):
     the budget, else 'plain' (the fallback); returns (form, estimate)."""
     est = judge(net, eng_red, eng_plain, pos, kappa, alpha_red)
     return ("learned" if est <= budget else "plain"), est
+
+
+class LearnedForce:
+    """The host driver's switch between the learned form (the reduced setting
+    with the network's correction attached) and plain P3M.
+
+    ``decide(pos)`` judges the corrected reduced force against the plain one
+    (choose_form).  Within the budget the network is attached to the reduced
+    engine and force() / md_run() run there; otherwise (the fallback) the
+    network is detached and every call is the plain engine's own call, with
+    the plain engine's arguments and buffers, so the caller gets the plain
+    path's forces.  Until decide() has run, calls are refused."""
+
+    def __init__(self, net, eng_red, eng_plain, kappa, alpha_red, budget=1e-5):
+        self.net, self.eng_red, self.eng_plain = net, eng_red, eng_plain
+        self.kappa, self.alpha_red, self.budget = float(kappa), float(alpha_red), float(budget)
+        self.form = None
+        self.estimate = None
+
+    def decide(self, pos):
+        self.net.detach(self.eng_red)  # the judge adds the correction itself
+        form, est = choose_form(self.net, self.eng_red, self.eng_plain, pos, self.kappa, self.alpha_red,
+                                self.budget)
+        if form == "learned":
+            self.net.attach(self.eng_red)
+        self.form, self.estimate = form, est
+        return form, est
+
+    @property
+    def engine(self):
+        if self.form is None:
+            raise RuntimeError("LearnedForce: call decide(pos) before stepping")
+        return self.eng_red if self.form == "learned" else self.eng_plain
+
+    def force(self, pos, *args, **kw):
+        return self.engine.force(pos, *args, **kw)
+
+    def md_run(self, pos, vel, acc, nsteps, dt, *args, **kw):
+        return self.engine.md_run(pos, vel, acc, nsteps, dt, *args, **kw)

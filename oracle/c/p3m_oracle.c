@@ -395,6 +395,8 @@ int orc_gf_opt(double kappa, double Gew, int p, int mx_max, int my_max, int mz_m
     double kappa_sq = kappa * kappa, Gew_sq = Gew * Gew;
     int nx_mid = Mx / 2, ny_mid = My / 2, nz_mid = Mz / 2; /* (M-1)/2 for odd M */
     double A_pm = 0.0;
+    /* modes are independent; only A_pm's summation order depends on the threads */
+#pragma omp parallel for reduction(+ : A_pm) schedule(static)
     for (int nz = 0; nz < Mz; ++nz) {
         int nz_sh = nz - nz_mid;
         double kz = 2 * M_PI * nz_sh / Lz;

@@ -137,11 +137,14 @@ def fft3d(c, sign):
     return c
 
 
-def gf_opt(kappa, Gew, p, m_max, M, L):
+def gf_opt(kappa, Gew, p, m_max, M, L, threads=None):
+    """threads: None runs the serial checker; an int the OpenMP build (same
+    G_k; A_pm summed in another order)."""
     h = L / M
     G_k = np.zeros((M, M, M))
     A = np.zeros(1)
-    _check(lib().orc_gf_opt(kappa, Gew, p, m_max, m_max, m_max, M, M, M, h, h, h, L, L, L,
+    L_ = lib() if threads is None else lib_omp(threads)[0]
+    _check(L_.orc_gf_opt(kappa, Gew, p, m_max, m_max, m_max, M, M, M, h, h, h, L, L, L,
                             _ptr(G_k), _ptr(A)), "orc_gf_opt")
     return G_k, float(A[0])
 

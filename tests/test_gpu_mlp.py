@@ -149,3 +149,54 @@ def test_trained_network_reduces_the_residual():
     assert form == "plain" and est > 1e-5
     form2, _ = T.choose_form(net, eng, plain, test[0]["pos"], 2.0, 0.6, 1.0)
     assert form2 == "learned"
+
+
+def test_fallback_switch_routes_to_the_plain_engine(setup):
+    """LearnedForce on the device engines: a network far outside the budget is
+    refused, the network is detached and the driver's forces are the plain
+    engine's.  Two plain contexts are not bitwise alike (the binning slots and
+    the FP64 charge deposits are atomics taken in arrival order), so the bar
+    is the plain path's own run-to-run spread: 1e-13 relative."""
+    from mdforce import mlp, mlp_train as T
+    from mdforce.engine import P3M
+    S = setup
+    N, pos = S["N"], S["pos"]
+    L = (4 * math.pi * N / 3) ** (1 / 3)
+    red = S["e"]
+    net = mlp.PairMLP(N, 400000, S["meta"])
+    net.set_params(mlp.init_params(5, out_std=1.0))
+    plain = P3M(N, 32, L, 2.0, 0.7, 6.0)
+    plain.build_green()
+    plain2 = P3M(N, 32, L, 2.0, 0.7, 6.0)
+    plain2.build_green()
+    drv = T.LearnedForce(net, red, plain, 2.0, 0.86, budget=1e-5)
+    form, est = drv.decide(pos)
+    print(f"[fallback] judged {est:.3e} -> {form}")
+    assert form == "plain" and red._mlp is None
+    a_drv = drv.force(pos, None, af=1.0)[0].clone()
+    a_ref = plain2.force(pos, None, af=1.0)[0].clone()
+    torch.cuda.synchronize()
+    d = float(torch.sqrt(((a_drv - a_ref) ** 2).sum(1).mean() / (a_ref ** 2).sum(1).mean()))
+    assert d <= 1e-13
+    # three NVE steps through the driver follow the plain context's steps
+    p1, p2 = pos.clone(), pos.clone()
+    v1 = torch.zeros_like(p1)
+    v2 = torch.zeros_like(p2)
+    e1, _ = drv.md_run(p1, v1, a_drv, 3, 0.05)
+    e2, _ = plain2.md_run(p2, v2, a_ref, 3, 0.05)
+    torch.cuda.synchronize()
+    dp = (p1 - p2).abs()
+    assert float(torch.minimum(dp, L - dp).max()) <= 1e-12 * L
+    np.testing.assert_allclose(e1.cpu().numpy(), e2.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    # within a loose budget the network is attached and its correction enters
+    drv.budget = 1e9
+    assert drv.decide(pos)[0] == "learned" and red._mlp is net
+    a_l = drv.force(pos, None, af=1.0)[0].clone()
+    mlp.PairMLP.detach(red)
+    a_r = red.force(pos, None, af=1.0)[0].clone()
+    corr = net.correction(red, pos)
+    torch.cuda.synchronize()
+    assert float(((a_l - a_r - corr) ** 2).sum().sqrt() / (corr ** 2).sum().sqrt()) <= 1e-5
+    net.close()
+    plain.close()
+    plain2.close()
