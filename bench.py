@@ -70,8 +70,9 @@ def parse():
                     help="NVE steps of the single-thread CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-core baseline (0 = all)")
     ap.add_argument("--skin", type=float, default=0.05,
-                    help="Verlet pair-list skin of the PP stage (0 = off); the cells of rc + skin set the "
-                         "skin actually used (the largest they allow)")
+                    help="Verlet pair-list skin of the PP stage with --no-autotune (0 = off); the cells of "
+                         "rc + skin set the skin actually used (the largest they allow).  The autotuner picks "
+                         "the skin, PP waves and PM block shapes by timing")
     ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "drift", "sqw"), default="n1e6",
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
@@ -650,8 +651,16 @@ def main():
         eng = P3M(N, M, L, a.kappa, a.alpha, a.rc, precision=precision, device=dev)
         G_full, _ = eng.build_green(6, 3, want_full=True)
         err = force_error(eng)
-        if a.skin > 0:
+        tl = None
+        if a.autotune:
+            # skin (and cell size), PP waves and PM block shapes by timing on this state
+            from mdforce import autotune
+            tl = autotune.tune_launch(eng, start_pos, start_vel, a.dt, steps=20)
+            print(f"[bench] {precision}: launch tuned to {tl['requested']} ({tl['ms_per_step']:.4f} ms/step in "
+                  f"{len(tl['timings'])} timings)", file=sys.stderr, flush=True)
+        elif a.skin > 0:
             eng.set_skin(a.skin)
+        skin = tl["skin"] if tl else a.skin
         pp_cfg = eng.config()
         pos = start_pos.clone()
         vel = start_vel.clone()
@@ -664,7 +673,7 @@ def main():
         energies = torch.empty((a.steps, 4), dtype=torch.float64, device=dev)
         nqt = torch.empty((a.steps, a.nq, 3), dtype=torch.complex128, device=dev)
         torch.cuda.synchronize()
-        ls0 = eng.list_stats() if a.skin > 0 else None
+        ls0 = eng.list_stats() if skin > 0 else None
         eng.profile_start(events=16 * (a.steps + 4))
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
@@ -679,9 +688,10 @@ def main():
         stages = eng.profile_read()
         eng.profile_stop()
         lists = None
-        if a.skin > 0:
+        launch = eng.launch_config()
+        if skin > 0:
             ls1 = eng.list_stats()
-            lists = {"skin_requested": a.skin, "cells_x": ls1["cells_x"],
+            lists = {"skin_requested": skin, "cells_x": ls1["cells_x"],
                      "skin_effective": L / ls1["cells_x"] - a.rc, "list_cap": ls1["list_cap"],
                      "builds_timed": ls1["builds"] - ls0["builds"],
                      "list_steps_timed": ls1["list_steps"] - ls0["list_steps"]}
@@ -689,7 +699,9 @@ def main():
         drift = float(np.max(np.abs(en_h[:, 0] - en_h[0, 0])) / (N * T))
         eng.close()
         return {"gpu_ms": gpu_ms, "wall": wall, "stages": stages, "err": err, "drift": drift, "G_full": G_full,
-                "pp_config": pp_cfg, "lists": lists,
+                "pp_config": pp_cfg, "lists": lists, "launch": launch,
+                "launch_tuning": ({"chosen": tl["requested"], "timings_ms": [[w, ms] for w, ms in tl["timings"]]}
+                                  if tl else None),
                 "acc0": acc_init, "value": N * a.steps / (gpu_ms / 1e3)}
 
     forms = {"both": ("fp64", "mixed"), "chosen": ("mixed",), "plain": ("fp64",)}[a.form]
@@ -727,7 +739,8 @@ def main():
         form_cfg["chosen" if f == "mixed" else "plain"] = {
             "particle_steps_per_s": r["value"], "ms_per_step": r["gpu_ms"] / a.steps, "force_error": r["err"],
             "energy_drift_per_NT": r["drift"], "stage_ms": stage_ms(r["stages"]), "wall_s": r["wall"],
-            "pp_config": r["pp_config"], "pair_lists": r["lists"]}
+            "pp_config": r["pp_config"], "pair_lists": r["lists"], "launch": r["launch"],
+            "launch_tuning": r["launch_tuning"]}
     if len(res) == 2:
         form_cfg["chosen_over_plain"] = res["mixed"]["value"] / res["fp64"]["value"]
     G_full = R["G_full"]

@@ -171,6 +171,21 @@ int mdf_p3m_stats(mdf_p3m_ctx *ctx, int *out2);
  * capacity in particles, pieces of the pair-term table, PP LDS bytes. */
 int mdf_p3m_config(mdf_p3m_ctx *ctx, int *out4);
 
+/* Launch choices (no reference counterpart; for the autotuner, which picks
+ * them by timing).  pp_waves: waves per PP workgroup, 4, 8, 12 or 16;
+ * assign_shape, gather_shape: PM block shapes in sub-cells per axis,
+ * 1 = 8x8x8, 2 = 6x6x6, 3 = 4x4x4, 4 = 2x2x2, 5 = 1x2x2, 6 = 1x1x2,
+ * 7 = 1x1x1 (assignment: 1..4).  0 leaves a choice to the context (its
+ * occupancy and LDS rules).  A choice whose LDS does not fit, or whose PP
+ * tile would hold fewer than 0.8 of the mean tile, is refused with
+ * MDF_EINVAL and the context is unchanged.  The context is rebuilt in place
+ * as by mdf_p3m_set_skin (skin, Green function and network carry over).
+ * Forces do not depend on these choices beyond summation order. */
+int mdf_p3m_set_launch(mdf_p3m_ctx *ctx, int pp_waves, int assign_shape, int gather_shape);
+/* host pointer, 6 ints: PP waves, assignment and gather shapes (as above),
+ * assignment and gather workgroups per launch, PP LDS tile capacity. */
+int mdf_p3m_launch_config(mdf_p3m_ctx *ctx, int *out6);
+
 /* Replica ensembles (no single reference counterpart: the reference runs
  * each seeded replica of yukawa_MD_p3m_ord6.py:195-219 as its own process
  * and averages S(q,w) afterwards, s_of_q_w_avg.py).  A batch owns one

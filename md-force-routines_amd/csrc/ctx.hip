@@ -147,7 +147,9 @@ namespace mdf {
 // A context whose cell grid is sized for rc + skin (skin = 0: the reference's
 // cells of L / floor(L / rc), yukawa_pp.py:17-23); the pair terms and the
 // cutoff stay rc.  skin > 0 also allocates the Verlet pair lists.
-static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
+// req: PP waves per workgroup, assignment and gather block shapes (1-based
+// indices into the shape list below); 0 leaves the choice to this function.
+static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], mdf_p3m_ctx **out) {
     if (!p || !out) {
         set_error("mdf_p3m_create: null argument");
         return MDF_EINVAL;
@@ -155,6 +157,15 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
     *out = nullptr;
     if (!(skin >= 0.0) || !(skin < 1e300)) {
         set_error("pair-list skin %g: need a finite skin >= 0", skin);
+        return MDF_EINVAL;
+    }
+    if (req[0] != 0 && req[0] != 4 && req[0] != 8 && req[0] != 12 && req[0] != 16) {
+        set_error("PP waves per workgroup %d: need 4, 8, 12 or 16 (0 = automatic)", req[0]);
+        return MDF_EINVAL;
+    }
+    if (req[1] < 0 || req[1] > 4 || req[2] < 0 || req[2] > 7) {
+        set_error("PM block shapes %d / %d: need 0..4 (assignment) and 0..7 (gather), 0 = automatic", req[1],
+                  req[2]);
         return MDF_EINVAL;
     }
     const double rcg = p->rc + skin;  // the cells hold every pair within rc + skin
@@ -266,7 +277,9 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
     // (multipass).  The 12-wave kernel packs its tile to 24 B a particle
     // (32 B otherwise): at rc = 2w and unit density (2568-particle tiles) that
     // is two workgroups of 12 waves, against one of 16.
-    if (mixed) {
+    if (req[0]) {
+        c->pp_waves = req[0];
+    } else if (mixed) {
         int best = 16, best_waves = 0;
         for (int nw : {8, 12, 16}) {
             const size_t l = pp_lds_bytes(c->tile_max, pp_table_pieces(c), true, nw) + 1024;
@@ -300,6 +313,14 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
     if (c->tile_max < 256) {
         delete c;
         set_error("PP pair table too large for LDS (kappa=%g alpha=%g rc=%g)", p->kappa, p->alpha, p->rc);
+        return MDF_EINVAL;
+    }
+    if (req[0] && c->tile_max < ((int)(0.8 * mean_tile) & ~3)) {
+        // a chosen wave count may not shrink the tile below most cells' size
+        // (each overfull tile takes a second pass over its rows)
+        const int tmx = c->tile_max;
+        delete c;
+        set_error("PP with %d waves: LDS tile of %d particles for a mean tile of %.0f", req[0], tmx, mean_tile);
         return MDF_EINVAL;
     }
 
@@ -337,12 +358,21 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
             int k = 0;
             int B[3];
             while (k < 3 && extent(shapes[k], B) * 8 > 64 * 1024) ++k;
+            if (req[1]) {
+                k = req[1] - 1;
+                if (extent(shapes[k], B) * 8 > 128 * 1024) {
+                    delete c;
+                    set_error("assignment block shape %d: its brick exceeds the 128 KB of LDS", req[1]);
+                    return MDF_EINVAL;
+                }
+            }
             const size_t nn = extent(shapes[k], B);
             for (int d = 0; d < 3; ++d) {
                 c->brick.sb_assign[d] = shapes[k][d];
                 c->brick.B[d] = B[d];
             }
             c->brick.lds_assign = nn * 8 <= 128 * 1024 ? 1 : 0;
+            c->shape_assign = k + 1;
         }
         {
             // 156 KB: with the gather's < 1 KB of static LDS within the 160 KB
@@ -352,14 +382,24 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
             int k = 0;
             int B[3];
             while (k < 7 && extent(shapes[k], B) * 3 * es > 156 * 1024) ++k;
-            const bool fits = k < 7;
+            bool fits = k < 7;
             if (!fits) k = 3;
+            if (req[2]) {
+                k = req[2] - 1;
+                if (extent(shapes[k], B) * 3 * es > 156 * 1024) {
+                    delete c;
+                    set_error("gather block shape %d: its field brick exceeds the 156 KB of LDS", req[2]);
+                    return MDF_EINVAL;
+                }
+                fits = true;
+            }
             extent(shapes[k], B);
             for (int d = 0; d < 3; ++d) {
                 c->brick.sb_gather[d] = shapes[k][d];
                 c->brick.Bg[d] = B[d];
             }
             c->brick.lds_gather = fits ? 1 : 0;
+            c->shape_gather = k + 1;
         }
         c->brick.nblock_assign = nblocks(c->brick.sb_assign);
         c->brick.nblock_gather = nblocks(c->brick.sb_gather);
@@ -458,6 +498,10 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
         free_ctx(c);
         return r;
     }
+    c->req_waves = req[0];
+    c->req_assign = req[1];
+    c->req_gather = req[2];
+    c->skin_req = skin;
     if (hipDeviceSynchronize() != hipSuccess) {
         set_error("device synchronisation failed after create");
         free_ctx(c);
@@ -471,26 +515,28 @@ static int create_ctx(const mdf_p3m_params *p, double skin, mdf_p3m_ctx **out) {
 
 extern "C" {
 
-int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) { return create_ctx(p, 0.0, out); }
+int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
+    static const int automatic[3] = {0, 0, 0};
+    return create_ctx(p, 0.0, automatic, out);
+}
 
-int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin) {
-    if (!ctx) {
-        set_error("mdf_p3m_set_skin: null context");
-        return MDF_EINVAL;
-    }
-    if (skin == ctx->vl_skin) return MDF_OK;
-    // a new context with the cells of rc + skin; the Green function and the
-    // attached network carry over, then the two swap contents so that the
-    // caller's handle stays valid
+}  // extern "C"
+
+namespace mdf {
+// Rebuild a context with another skin or launch choice: a new context is
+// created, the Green function and the attached network carry over, then the
+// two swap contents so that the caller's handle stays valid.  On failure the
+// context is unchanged.
+static int rebuild_ctx(mdf_p3m_ctx *ctx, double skin, const int req[3], const char *who) {
     mdf_p3m_ctx *nc = nullptr;
-    MDF_TRY(create_ctx(&ctx->prm, skin, &nc));
+    MDF_TRY(create_ctx(&ctx->prm, skin, req, &nc));
     if (ctx->have_green) {
         hipError_t e = hipMemcpy(nc->green, ctx->green, ctx->nspec * sizeof(double), hipMemcpyDeviceToDevice);
         if (e == hipSuccess && ctx->greenf)
             e = hipMemcpy(nc->greenf, ctx->greenf, ctx->nspec * sizeof(float), hipMemcpyDeviceToDevice);
         if (e != hipSuccess) {
             free_ctx(nc);
-            set_error("mdf_p3m_set_skin: Green function copy failed (%s)", hipGetErrorString(e));
+            set_error("%s: Green function copy failed (%s)", who, hipGetErrorString(e));
             return MDF_EHIP;
         }
         nc->have_green = true;
@@ -502,6 +548,44 @@ int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin) {
     std::memcpy((void *)ctx, (void *)nc, sizeof tmp);
     std::memcpy((void *)nc, (void *)&tmp, sizeof tmp);
     free_ctx(nc);
+    return MDF_OK;
+}
+}  // namespace mdf
+
+extern "C" {
+
+int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin) {
+    if (!ctx) {
+        set_error("mdf_p3m_set_skin: null context");
+        return MDF_EINVAL;
+    }
+    if (skin == ctx->skin_req) return MDF_OK;
+    const int req[3] = {ctx->req_waves, ctx->req_assign, ctx->req_gather};
+    return rebuild_ctx(ctx, skin, req, "mdf_p3m_set_skin");
+}
+
+int mdf_p3m_set_launch(mdf_p3m_ctx *ctx, int pp_waves, int assign_shape, int gather_shape) {
+    if (!ctx) {
+        set_error("mdf_p3m_set_launch: null context");
+        return MDF_EINVAL;
+    }
+    if (pp_waves == ctx->req_waves && assign_shape == ctx->req_assign && gather_shape == ctx->req_gather)
+        return MDF_OK;
+    const int req[3] = {pp_waves, assign_shape, gather_shape};
+    return rebuild_ctx(ctx, ctx->skin_req, req, "mdf_p3m_set_launch");
+}
+
+int mdf_p3m_launch_config(mdf_p3m_ctx *ctx, int *out6) {
+    if (!ctx || !out6) {
+        set_error("mdf_p3m_launch_config: null argument");
+        return MDF_EINVAL;
+    }
+    out6[0] = ctx->pp_waves;
+    out6[1] = ctx->shape_assign;
+    out6[2] = ctx->shape_gather;
+    out6[3] = ctx->brick.nblock_assign;
+    out6[4] = ctx->brick.nblock_gather;
+    out6[5] = ctx->tile_max;
     return MDF_OK;
 }
 

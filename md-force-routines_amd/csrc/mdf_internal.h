@@ -96,7 +96,13 @@ struct mdf_p3m_ctx {
     int nblock;         // total PP blocks
     int tile_max;       // LDS tile capacity (particles)
     size_t pp_lds;      // dynamic LDS bytes of the PP kernel
-    int pp_waves;       // waves per PP workgroup (4, or 16 for tiles above 64 KB)
+    int pp_waves;       // waves per PP workgroup (4, 8, 12 or 16)
+    int shape_assign;   // PM block shapes in use (1-based index into the shape list of create)
+    int shape_gather;
+    // launch choices asked for by mdf_p3m_set_launch (0 = chosen at create) and
+    // the skin asked for by mdf_p3m_set_skin; kept when the context is rebuilt
+    int req_waves, req_assign, req_gather;
+    double skin_req;
     size_t nmesh;       // Mx*My*Mz
     size_t nspec;       // Mz*My*Mxh
     bool have_green;

@@ -71,6 +71,8 @@ SIGNATURES = {
     "mdf_p3m_set_skin": [_vp, _d],
     "mdf_p3m_list_stats": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_config": [_vp, ct.POINTER(ct.c_int)],
+    "mdf_p3m_set_launch": [_vp, ct.c_int, ct.c_int, ct.c_int],
+    "mdf_p3m_launch_config": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_profile": [_vp, _i, _vp, _i],
     "mdf_batch_create": [_i, ct.POINTER(P3MParams), _i, ct.POINTER(_vp)],
     "mdf_batch_destroy": [_vp],

@@ -19,6 +19,11 @@ solve per mesh node at M = 128, scaled by log2(M^3)/21), t_0 = 2e-5 (launch
 overheads per step of a replica).
 
 The PP cell list needs L/rc >= 3, so rc <= L/3.
+
+tune_launch then picks, by timing NVE steps of the chosen setting on the
+system's own state, what the error does not depend on: the Verlet skin (and
+with it the PP cell size: the cells of rc + skin), the PP waves per
+workgroup and the PM block shapes of the assignment and the gather.
 """
 from __future__ import annotations
 
@@ -159,3 +164,86 @@ def _tune(pos, L, kappa, budget, meshes, rcs, alphas, precision, margin, ref, de
         if best is not None and best[1] <= margin * budget:
             return Choice(M, best[0], rc, best[1], cost, tried)
     raise NoCandidateError(f"no (mesh, alpha, rc) meets the force budget {budget} (N={N}, L={L:.3f}, kappa={kappa})")
+
+
+def skin_candidates(L, rc, cell_counts=3):
+    """0 (no pair lists) and, for the cell counts k = floor(L/rc), k - 1, ...,
+    the largest skin of k cells per axis: L/k - rc (cells of rc + skin)."""
+    Lm = min(L) if isinstance(L, (tuple, list)) else L
+    k0 = int(math.floor(Lm / rc))
+    out = [0.0]
+    for k in range(k0, max(k0 - cell_counts, 2), -1):
+        s = Lm / k - rc
+        if s > 1e-6:
+            out.append(s * (1.0 - 1e-9))
+    return out
+
+
+def time_steps(eng, pos, vel, dt, steps=20, warmup=2):
+    """ms per NVE step (md_run) of eng from a copy of (pos, vel)."""
+    p, v = pos.clone(), vel.clone()
+    acc, _ = eng.force(p)
+    acc = acc.clone()
+    if warmup:
+        eng.md_run(p, v, acc, warmup, dt)
+    s0 = torch.cuda.Event(enable_timing=True)
+    s1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s0.record()
+    eng.md_run(p, v, acc, steps, dt)
+    s1.record()
+    torch.cuda.synchronize()
+    return s0.elapsed_time(s1) / steps
+
+
+def tune_launch(eng, pos, vel, dt, steps=20, skins=None, min_gain=0.01, log=None):
+    """Launch choices of eng by measurement, one knob at a time: the skin
+    (skin_candidates; the cell size follows), then the PP waves per workgroup,
+    then the assignment block shape, then the gather block shape.  A choice
+    replaces the current one only when it is faster by more than min_gain.
+    Choices the context refuses (LDS) are skipped.  eng is left with the
+    chosen settings; returns a dict of them and every timing."""
+    if skins is None:
+        skins = skin_candidates(eng.box, eng.rc)
+    timings = []
+
+    def measure(what):
+        ms = time_steps(eng, pos, vel, dt, steps)
+        timings.append((what, ms))
+        if log:
+            log(f"[tune_launch] {what}: {ms:.4f} ms/step")
+        return ms
+
+    cur = {"skin": getattr(eng, "skin", 0.0), "pp_waves": 0, "assign_shape": 0, "gather_shape": 0}
+    eng.set_skin(cur["skin"])
+    eng.set_launch(0, 0, 0)
+    best = measure(dict(cur))
+
+    def try_knob(name, values, apply):
+        nonlocal best
+        in_use = eng.launch_config().get(name)
+        for val in values:
+            if val == cur[name] or (cur[name] == 0 and val == in_use):
+                continue
+            trial = dict(cur, **{name: val})
+            try:
+                apply(trial)
+            except MdforceError as err:
+                if err.code != MDF_EINVAL:
+                    raise
+                timings.append((trial, None))
+                continue
+            ms = measure(trial)
+            if ms < best * (1.0 - min_gain):
+                best = ms
+                cur.update(trial)
+        apply(cur)
+
+    try_knob("skin", skins, lambda c: eng.set_skin(c["skin"]))
+    launch = lambda c: eng.set_launch(c["pp_waves"], c["assign_shape"], c["gather_shape"])
+    try_knob("pp_waves", (4, 8, 12, 16), launch)
+    try_knob("assign_shape", (1, 2, 3, 4), launch)
+    try_knob("gather_shape", (1, 2, 3, 4, 5, 6, 7), launch)
+    cfg = eng.launch_config()
+    return {"skin": cur["skin"], "requested": dict(cur), "in_use": cfg, "ms_per_step": best,
+            "timings": [(w, ms) for w, ms in timings]}

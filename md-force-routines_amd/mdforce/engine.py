@@ -120,6 +120,20 @@ class P3M:
         check(self._lib.mdf_p3m_set_skin(self._ctx, float(skin)), "mdf_p3m_set_skin")
         self.skin = float(skin)
 
+    def set_launch(self, pp_waves=0, assign_shape=0, gather_shape=0):
+        """Launch choices (0 = the context's own): PP waves per workgroup (4, 8,
+        12, 16) and the PM block shapes (1 = 8^3 .. 4 = 2^3 sub-cells, gather
+        also 5 = 1x2x2, 6 = 1x1x2, 7 = 1x1x1).  Forces are unchanged up to
+        summation order."""
+        check(self._lib.mdf_p3m_set_launch(self._ctx, int(pp_waves), int(assign_shape), int(gather_shape)),
+              "mdf_p3m_set_launch")
+
+    def launch_config(self):
+        out = (ct.c_int * 6)()
+        check(self._lib.mdf_p3m_launch_config(self._ctx, out), "mdf_p3m_launch_config")
+        return {"pp_waves": out[0], "assign_shape": out[1], "gather_shape": out[2], "assign_blocks": out[3],
+                "gather_blocks": out[4], "tile_max": out[5]}
+
     def list_stats(self):
         out = (ct.c_int * 4)()
         check(self._lib.mdf_p3m_list_stats(self._ctx, out), "mdf_p3m_list_stats")
