@@ -67,13 +67,14 @@ struct Brick {
 };
 
 // FP32 pair table indexed by the float bits of s = r^2 (pair_table.hip): piece
-// p = (bits(s) >> shift) - base for s >= s_lo, t = tscale * f + toff with f the
-// float of s's low `shift` mantissa bits under the exponent of 1.
+// p = (bits(s) >> shift) - base for s >= s_lo; the local variable is f - tc,
+// with f the float of s's low `shift` mantissa bits under the exponent of 1
+// and tc the centre of f's range.
 struct SPieces {
     int npiece;
     unsigned base;
     int shift;
-    float s_lo, tscale, toff;
+    float s_lo, tc;
 };
 
 constexpr int kPMThreads = 512;

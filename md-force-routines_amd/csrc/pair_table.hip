@@ -154,6 +154,9 @@ namespace mdf {
 // piece the table holds, as degree-deg power series of t in [-1, 1),
 //   phi(s) = g1(r) / r      (pair energy)
 //   F(s)   = g2(r) / r^3    (force on i: F (x_i - x_j))
+// stored for the variable f - tc = t / 2^(m+1) (coefficient k times
+// 2^((m+1) k), exact in float): the kernel forms f - tc with one subtraction,
+// and Horner in it gives t's results bit for bit (power-of-two scalings).
 // fitted at the Chebyshev nodes in long double; the fit with coefficients
 // rounded to float is checked at 64 points per piece, with the same measure as
 // the r-indexed table: the errors of g1 = r phi and g2 = r^3 F against tol
@@ -232,12 +235,18 @@ int build_pair_table_s(double kappa, double alpha, double rc, int deg, double to
             ok = e1 <= (long double)tol * g1max && e2 <= (long double)tol * g2max;
         }
         if (ok) {
+            // f in [1, 1 + 2^-m): t = 2^(m+1) (f - tc), tc = 1 + 2^-(m+1)
+            for (int p = 0; p < P; ++p)
+                for (int k = 0; k < nc; ++k) {
+                    const double sc = std::ldexp(1.0, (m + 1) * k);
+                    tab[(size_t)p * nc + k].x *= sc;
+                    tab[(size_t)p * nc + k].y *= sc;
+                }
             sp.npiece = P;
             sp.base = base;
             sp.shift = shift;
             sp.s_lo = s_lo;
-            sp.tscale = (float)(1 << (m + 1));
-            sp.toff = -(float)((1 << (m + 1)) + 1);
+            sp.tc = 1.0f + std::ldexp(1.0f, -(m + 1));
             return MDF_OK;
         }
     }

@@ -82,7 +82,7 @@ struct PPArgs {
     const T2 *__restrict__ rtab;
     unsigned s_base;
     int s_shift;
-    float s_lo, s_tscale, s_toff;
+    float s_lo, s_tc;
     T rc2;                             // rc^2, the pair cutoff
     T rc2m;                            // candidate test radius^2 (rc slightly widened)
     // Verlet pair lists (null vl_flags: off).  vl_flags[3] is the step's mode:
@@ -226,20 +226,25 @@ __device__ __forceinline__ void pp_pair_masked(const PPArgs<T> &P, const typenam
 // take the r-indexed table from global memory.  Contributions of pairs that
 // are not ok, outside the cutoff or at r = 0 are selected away (no branch
 // except the rare close-pair one).
+// LIST: a list entry (pp_body's list mode), which is never i itself: lists hold
+// a far dummy slot in place of i and as padding, which the cutoff drops, so
+// only s < rc^2 is tested (and s > 0 only on the rare close-pair path).
+template <bool LIST = false>
 __device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, const float2 *__restrict__ stab, bool ok, float dx,
                                           float dy, float dz, float &fx, float &fy, float &fz, float &u) {
     constexpr int D = kChebDegF;
     const float s2 = dx * dx + dy * dy + dz * dz;
-    const bool in = ok && s2 < P.rc2 && s2 > 0.0f;
+    const bool in = LIST ? s2 < P.rc2 : ok && s2 < P.rc2 && s2 > 0.0f;
     const unsigned b = __float_as_uint(s2);
-    int p = (int)(b >> P.s_shift) - (int)P.s_base;
-    p = min(max(p, 0), P.ntab - 1);
-    const float t = fmaf(__uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u), P.s_tscale, P.s_toff);
+    // piece index, clamped with one unsigned min: s < s_lo wraps to the top
+    // piece, whose values the close-pair path below replaces
+    const unsigned p = min((b >> P.s_shift) - P.s_base, (unsigned)P.ntab - 1u);
+    const float t = __uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u) - P.s_tc;
     using LdsF2 = __attribute__((address_space(3))) const float2;
-    LdsF2 *c = (LdsF2 *)stab + __umul24((unsigned)p, D + 1);
+    LdsF2 *c = (LdsF2 *)stab + __umul24(p, D + 1);
     float phi, F;
     horner2<float>(c, t, phi, F);
-    if (in && s2 < P.s_lo) {
+    if (LIST ? s2 < P.s_lo : in && s2 < P.s_lo) {
         const float ir = rsqrt_t<float>(s2);
         const float r = s2 * ir;
         const int q = min((int)(r * P.inv_piece_w), P.npiece - 1);
@@ -248,6 +253,7 @@ __device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, const float2 *
         horner2<float>(P.rtab + q * (D + 1), tq, g1, g2);
         phi = g1 * ir;
         F = g2 * ir * ir * ir;
+        if (LIST && !(s2 > 0.0f)) phi = F = 0.0f;  // coincident ions: no pair term, as the sweep
     }
     u += in ? phi : 0.0f;
     const float Fm = in ? F : 0.0f;
@@ -292,6 +298,10 @@ __device__ __forceinline__ float half_wave_sum(float v) {
     return __shfl(v, kGL - 1, kGL);
 }
 
+// Slots a list of cnt entries occupies with its padding: whole chunks of
+// 2 kGL entries and at least two padding slots (one list word) at the end.
+__device__ __forceinline__ int list_padded(int cnt) { return (cnt + 2 + 2 * kGL - 1) & ~(2 * kGL - 1); }
+
 // NW waves per workgroup share the tile: 4 for tiles up to 64 KB, 16 for
 // larger ones (large rc), where one workgroup per CU would otherwise leave a
 // single wave per SIMD.
@@ -321,6 +331,11 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     __shared__ int2 rng[8][5];   // per home sub-cell: LDS range of each of its 5 stencil planes
     __shared__ double red[NW];
 
+    // LISTS: the last tile slot is a dummy far from every particle; lists
+    // hold it in place of i itself and as padding to whole chunks, and the
+    // cutoff test drops it
+    const int kFar = T - 1;
+    const int Tst = LISTS ? T - 1 : T;  // particles a pass can stage
     const Geom &g = P.g;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int grp = lane / kGL, gl = lane % kGL;
@@ -330,6 +345,15 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     const int bz = bid / (g.nblk[0] * g.nblk[1]);
 
     for (int t = tid; t < P.ntab * (D + 1); t += blockDim.x) tab[t] = P.cheb[t];
+    if (LISTS && tid == 0) {
+        if constexpr (kRel) {
+            th4[kFar] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+            if constexpr (kPack) tl2[kFar] = make_float2(0.0f, 0.0f);
+            else tl4[kFar] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            tx[kFar] = ty[kFar] = tz[kFar] = (R)1e150;
+        }
+    }
     // origin of the tile coordinates: 0, or the home block centre (mixed)
     const double ox = kRel ? (2 * bx + 1) * g.w[0] : 0.0;
     const double oy = kRel ? (2 * by + 1) * g.w[1] : 0.0;
@@ -400,7 +424,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     while (r0 < 36) {
         const int base = cum[6 * r0];
         int r1 = r0;
-        while (r1 < 36 && cum[6 * (r1 + 1)] - base <= T) ++r1;
+        while (r1 < 36 && cum[6 * (r1 + 1)] - base <= Tst) ++r1;
         if (r1 == r0) {  // a single row larger than the LDS tile: unsupported density
             if (tid == 0) atomicOr(&P.stats[2], 1);
             r0 = r0 + 1;
@@ -545,15 +569,16 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             auto pair_at = [&](int jj) {
                 R dx, dy, dz;
                 disp(jj, dx, dy, dz);
-                if constexpr (kRel) pp_pair_f(P, tab, true, dx, dy, dz, fx, fy, fz, u);
+                if constexpr (kRel) pp_pair_f<false>(P, tab, true, dx, dy, dz, fx, fy, fz, u);
                 else pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
             };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
             int lc0 = 0, lc1 = 0;  // list entries written for the i of half 0 / half 1 (build)
             unsigned short *const lst = build || use_list ? P.vl_list + (size_t)gi * P.vl_cap : nullptr;
             // build: the queued j of this half go to its i's list, slots lc .. lc + cnt - 1
+            // (i itself, queued by the FP32 sweep, as the dummy slot)
             auto list_put = [&](int jj, int at) {
-                if (at < P.vl_cap) lst[at] = (unsigned short)jj;
+                if (at < P.vl_cap) lst[at] = (unsigned short)(jj == islot ? kFar : jj);
             };
             if (use_list) {
                 // the step's pairs from i's list: every j within the list radius at
@@ -561,32 +586,32 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 // consecutive entries (one dword) of each chunk of 2 kGL; the
                 // chunks are loaded two ahead of their use, so that the list reads
                 // from memory overlap the pair terms of the chunks before
+                // a list is padded with the dummy slot to a whole chunk past its
+                // last entry, so its last chunk word holds padding only; a lane
+                // past i's padded end re-reads that word (the load index is
+                // clamped: the load is unconditional, and its wait sits where
+                // the value is used)
                 const int cnt = have_i ? P.vl_cnt[gi] : 0;
-                const int nchunk = (max(readlane_i(cnt, 0), readlane_i(cnt, kGL)) + 2 * kGL - 1) / (2 * kGL);
-                const unsigned *lw = (const unsigned *)lst;  // vl_cap is a multiple of 32: dword aligned
-                // the load itself is unconditional (its index clamped to i's last
-                // word, or word 0 of an empty list) so that the wait for it can be
-                // placed where its value is used
-                const int wlast = cnt > 0 ? (cnt - 1) >> 1 : 0;
+                const int cntp = list_padded(cnt);
+                const int nchunk = max(readlane_i(cntp, 0), readlane_i(cntp, kGL)) / (2 * kGL);
+                const unsigned *lw = (const unsigned *)lst;  // vl_cap is a multiple of 64: dword aligned
+                const int wlast = (cntp >> 1) - 1;
                 auto chunk_word = [&](int c) -> unsigned { return lw[min(c * kGL + gl, wlast)]; };
                 unsigned wa = chunk_word(0), wb = chunk_word(1);
                 for (int c = 0; c < nchunk; ++c) {
                     const unsigned w = wa;
                     wa = wb;
                     wb = chunk_word(c + 2);
-                    const int k = 2 * (c * kGL + gl);
-                    const bool ok0 = k < cnt, ok1 = k + 1 < cnt;
-                    // past i's count a lane reads i's own slot (inside the tile) and drops it
-                    const int j0 = ok0 ? (int)(w & 0xffffu) : islot, j1 = ok1 ? (int)(w >> 16) : islot;
+                    const int j0 = (int)(w & 0xffffu), j1 = (int)(w >> 16);
                     R dx0, dy0, dz0, dx1, dy1, dz1;
                     disp(j0, dx0, dy0, dz0);
                     disp(j1, dx1, dy1, dz1);
                     if constexpr (kRel) {
-                        pp_pair_f(P, tab, ok0, dx0, dy0, dz0, fx, fy, fz, u);
-                        pp_pair_f(P, tab, ok1, dx1, dy1, dz1, fx, fy, fz, u);
+                        pp_pair_f<true>(P, tab, true, dx0, dy0, dz0, fx, fy, fz, u);
+                        pp_pair_f<true>(P, tab, true, dx1, dy1, dz1, fx, fy, fz, u);
                     } else {
-                        pp_pair_masked<R>(P, tab, ok0, dx0, dy0, dz0, fx, fy, fz, u);
-                        pp_pair_masked<R>(P, tab, ok1, dx1, dy1, dz1, fx, fy, fz, u);
+                        pp_pair_masked<R>(P, tab, true, dx0, dy0, dz0, fx, fy, fz, u);
+                        pp_pair_masked<R>(P, tab, true, dx1, dy1, dz1, fx, fy, fz, u);
                     }
                 }
             }
@@ -687,10 +712,17 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 if (build) list_put(jj, (grp ? lc1 : lc0) + gl);
                 pair_at(jj);
             }
-            if (build && gl == 0 && have_i) {
+            if (build && have_i) {
                 const int total = (grp ? lc1 : lc0) + qmine;
-                P.vl_cnt[gi] = min(total, P.vl_cap);
-                if (total > P.vl_cap) atomicOr(&P.vl_flags[1], 1);
+                const int pend = list_padded(total);
+                // dummy padding of the list's last chunk (two slots a lane)
+                if (pend <= P.vl_cap) {
+                    for (int at = total + gl; at < pend; at += kGL) lst[at] = (unsigned short)kFar;
+                }
+                if (gl == 0) {
+                    P.vl_cnt[gi] = min(total, P.vl_cap);
+                    if (pend > P.vl_cap) atomicOr(&P.vl_flags[1], 1);
+                }
             }
             double sfx, sfy, sfz, su;
             if constexpr (sizeof(R) == 4) {
@@ -798,8 +830,7 @@ static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, bool l
     P.s_base = c->sp.base;
     P.s_shift = c->sp.shift;
     P.s_lo = c->sp.s_lo;
-    P.s_tscale = c->sp.tscale;
-    P.s_toff = c->sp.toff;
+    P.s_tc = c->sp.tc;
     const double rcm = c->g.rc * (1.0 + 1e-12);
     P.rc2 = (R)(c->g.rc * c->g.rc);
     P.rc2m = sizeof(R) == 4 ? P.rc2 : (R)(rcm * rcm);

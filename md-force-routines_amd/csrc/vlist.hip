@@ -165,13 +165,14 @@ int vlist_alloc(mdf_p3m_ctx *c, double skin) {
     vlist_free(c);
     if (!(skin > 0.0)) return MDF_OK;
     // entries a particle: its neighbours within rc + skin at the mean density
-    // of n_max particles, with room for 40% more and 64 (rounded to 32)
+    // of n_max particles, with room for 40% more and 64, rounded to the list
+    // chunk of 64 (the lists are padded to whole chunks)
     const double V = c->g.L[0] * c->g.L[1] * c->g.L[2];
     const double rl = c->g.rc + skin;
     const double mean = (double)c->prm.n_max / V * 4.18879020478639 * rl * rl * rl;
     int cap = (int)(1.4 * mean + 64.0);
-    cap = (cap + 31) & ~31;
-    if (cap > 65535) cap = 65535 & ~31;
+    cap = (cap + 63) & ~63;
+    if (cap > 65535) cap = 65535 & ~63;
     const size_t n = (size_t)c->prm.n_max;
     hipError_t e = hipSuccess;
     auto alloc = [&](void **p, size_t bytes) {
