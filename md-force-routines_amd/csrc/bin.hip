@@ -7,6 +7,9 @@
 //   1. count: sub-cell of each particle, slot = atomicAdd(count[cell])
 //   2. scan : exclusive prefix sum of the counts (and reset of the counts)
 //   3. place: sorted index = start[cell] + slot; sorted (x, y, z, Z) copy.
+// With Verlet pair lists (vlist.hip) the sorted order is frozen between list
+// builds: on a list step the three kernels return at once (the step's mode is
+// on the device) and the sorted copy holds the positions in the frozen order.
 // Positions are wrapped into [0, L) for the sorted copy (a position at
 // exactly L, which the reference's cell index leaves out of range, lands in
 // the periodic image at 0); in-range positions are copied bit for bit.
@@ -45,7 +48,7 @@ struct BinArgs {
     int *cell_start;
     double4 *spos;
     int *perm;
-    int *inv_perm;     // caller index -> sorted slot, or null
+    const int *mode;   // pair-list mode of the step (2: lists, binning skipped), or null
 };
 
 __device__ __forceinline__ void bin_count_body(const BinArgs &A, const Blk &b) {
@@ -56,7 +59,7 @@ __device__ __forceinline__ void bin_count_body(const BinArgs &A, const Blk &b) {
     int *__restrict__ part_cell = A.part_cell;
     int *__restrict__ part_slot = A.part_slot;
     const int i = b.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (A.mode && *A.mode == 2)) return;
     int c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -98,6 +101,7 @@ __device__ __forceinline__ int block_incl_scan_256(int v, int *wsum) {
 }
 
 __device__ __forceinline__ void scan_tile_sums_body(const BinArgs &A, const Blk &b, int *wsum) {
+    if (A.mode && *A.mode == 2) return;  // uniform over the launch
     const int *__restrict__ cell_count = A.cell_count;
     const int ncell = A.ncell;
     const int base = b.x * kScanTile + threadIdx.x * 4;
@@ -120,6 +124,7 @@ __global__ __launch_bounds__(256) void k_scan_tile_sums_seg(const BinArgs *__res
 }
 
 __device__ __forceinline__ void scan_apply_body(const BinArgs &A, const Blk &b, int *wsum) {
+    if (A.mode && *A.mode == 2) return;  // uniform over the launch
     int *__restrict__ cell_count = A.cell_count;
     const int ncell = A.ncell;
     const int *__restrict__ tile_sum = A.tile;
@@ -175,14 +180,13 @@ __device__ __forceinline__ void bin_place_body(const BinArgs &A, const Blk &b) {
     double4 *__restrict__ spos = A.spos;
     int *__restrict__ perm = A.perm;
     const int i = b.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (A.mode && *A.mode == 2)) return;
     const int s = cell_start[part_cell[i]] + part_slot[i];
     const double x = wrap_box(pos[3 * i + 0], g.L[0]);
     const double y = wrap_box(pos[3 * i + 1], g.L[1]);
     const double z = wrap_box(pos[3 * i + 2], g.L[2]);
     spos[s] = make_double4(x, y, z, Z ? Z[i] : 1.0);
     perm[s] = i;
-    if (A.inv_perm) A.inv_perm[i] = s;
 }
 
 __global__ void k_bin_place(BinArgs A) { bin_place_body(A, blk_plain()); }
@@ -237,13 +241,14 @@ static BinArgs bin_args(mdf_p3m_ctx *c, const double *pos, const double *Z, int 
     A.cell_start = c->cell_start;
     A.spos = c->spos;
     A.perm = c->perm;
-    A.inv_perm = c->inv_perm;
+    A.mode = nullptr;
     return A;
 }
 
-int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s) {
+int bin_particles_mode(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, const int *mode, hipStream_t s) {
     if (n <= 0) return MDF_OK;
-    const BinArgs A = bin_args(c, pos, Z, n);
+    BinArgs A = bin_args(c, pos, Z, n);
+    A.mode = mode;
     const unsigned nb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, s, A);
     MDF_LAUNCHED();
@@ -254,6 +259,13 @@ int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hip
     hipLaunchKernelGGL(k_bin_place, dim3(nb), dim3(256), 0, s, A);
     MDF_LAUNCHED();
     return MDF_OK;
+}
+
+int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s) {
+    // with pair lists the sorted arrays are the lists' frozen order: binning
+    // outside a list-aware step replaces it, so the next step builds
+    if (c->vl_skin > 0.0) c->vl_force_build = 1;
+    return bin_particles_mode(c, pos, Z, n, nullptr, s);
 }
 
 // ---- segmented stages (replica batches) ----

@@ -87,7 +87,6 @@ static void free_ctx(mdf_p3m_ctx *c) {
                     c->twf[2],    c->part_pp,   c->part_pm,  c->part_red,   c->scal,       c->stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    if (c->inv_perm) (void)hipFree(c->inv_perm);
     if (c->stab) (void)hipFree(c->stab);
     vlist_free(c);
     delete c;
@@ -414,7 +413,6 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], md
     ALLOC(c->part_cell, (size_t)p->n_max);
     ALLOC(c->part_slot, (size_t)p->n_max);
     ALLOC(c->perm, (size_t)p->n_max);
-    ALLOC(c->inv_perm, (size_t)p->n_max);
     ALLOC(c->spos, (size_t)p->n_max);
     ALLOC(c->acc_pp_s, 3 * (size_t)p->n_max);
     ALLOC(c->rho, c->nmesh);
@@ -651,9 +649,8 @@ int mdf_p3m_force(mdf_p3m_ctx *ctx, const double *pos, const double *Z, int n, d
     MDF_TRY(need_green(ctx));
     hipStream_t s = (hipStream_t)stream;
     prof_mark(ctx, kStageBin, s);
-    MDF_TRY(bin_particles(ctx, pos, Z, n, s));
     ctx->vl_force_build = 1;  // positions from the caller: build the lists anew
-    MDF_TRY(vlist_prepare(ctx, pos, n, s));
+    MDF_TRY(bin_step(ctx, pos, Z, n, s));
     prof_mark(ctx, kStagePP, s);
     MDF_TRY(pp_forces(ctx, n, s));
     if (ctx->mlp) MDF_TRY(mlp_correct(ctx->mlp, ctx, n, s));
@@ -685,9 +682,8 @@ int mdf_pp_force(mdf_p3m_ctx *ctx, const double *pos, int n, double *acc_pp, dou
     }
     MDF_TRY(check_n(ctx, n));
     hipStream_t s = (hipStream_t)stream;
-    MDF_TRY(bin_particles(ctx, pos, nullptr, n, s));
     ctx->vl_force_build = 1;
-    MDF_TRY(vlist_prepare(ctx, pos, n, s));
+    MDF_TRY(bin_step(ctx, pos, nullptr, n, s));
     MDF_TRY(pp_forces(ctx, n, s));
     if (acc_pp) MDF_TRY(unpermute3(ctx, ctx->acc_pp_s, acc_pp, n, s));
     MDF_TRY(finalize_energy(ctx, n, 1.0, 1, 0, nullptr, s));

@@ -501,8 +501,7 @@ int md_step(mdf_p3m_ctx *c, double *pos, double *vel, double *acc, const double 
     prof_mark(c, kStageDrift, s);
     MDF_TRY(kick_drift(c, pos, vel, acc, n, dt, 1, nullptr, nullptr, nullptr, s));
     prof_mark(c, kStageBin, s);
-    MDF_TRY(bin_particles(c, pos, Z, n, s));
-    MDF_TRY(vlist_prepare(c, pos, n, s));
+    MDF_TRY(bin_step(c, pos, Z, n, s));
     prof_mark(c, kStagePP, s);
     MDF_TRY(pp_forces(c, n, s));
     if (c->mlp) MDF_TRY(mlp_correct(c->mlp, c, n, s));

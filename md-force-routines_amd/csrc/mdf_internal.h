@@ -149,17 +149,15 @@ struct mdf_p3m_ctx {
 
     // Verlet pair lists of the PP stage (vl_skin > 0: on; mdf_p3m_set_skin).
     // Lists hold, per particle, the tile slots of every j within rc + skin at
-    // the last build, in the frozen order (sorted order and cell starts of that
-    // build); they stay valid while no particle has moved more than skin / 2.
-    int *inv_perm;           // caller index -> current sorted slot (written by every binning)
+    // the last build, in the sorted order and cell starts of that build, which
+    // the binning then keeps (frozen) until the next build; they stay valid
+    // while no particle has moved more than skin / 2.  On a list step spos
+    // holds the step's positions in the frozen order.
     double vl_skin;
     int vl_cap;              // list entries per particle
     unsigned short *vl_list; // n_max * vl_cap
     int *vl_cnt;             // n_max
-    double4 *vl_pos;         // frozen order: this step's positions, unwrapped near vl_ref
     double4 *vl_ref;         // frozen order: positions at the build
-    int *vl_perm;            // frozen slot -> caller index
-    int *vl_cs;              // frozen cell starts (ncell + 1)
     int *vl_flags;           // [0] build requested, [1] last build incomplete, [3] mode of the step
                              // (1 build, 2 lists), [4] builds, [5] list steps (counts)
     int vl_force_build;      // host: the next list-mode step must build
@@ -185,7 +183,11 @@ enum {
 void prof_mark(mdf_p3m_ctx *c, int stage, hipStream_t s);
 
 // ----------------------------------------------------------- launchers --
+// Counting sort of the positions into the sorted copy.  bin_particles: always
+// (with pair lists on, the next list-aware step then builds);
+// bin_particles_mode: skipped on the device when *mode == 2 (a list step).
 int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s);
+int bin_particles_mode(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, const int *mode, hipStream_t s);
 int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s);
 int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s);
 int pm_solve(mdf_p3m_ctx *c, hipStream_t s);
@@ -248,11 +250,13 @@ int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int 
 // Copy a sorted (n,3) array back to the caller's particle order.
 int unpermute3(mdf_p3m_ctx *c, const double *src_sorted, double *dst, int n, hipStream_t s);
 
-// Verlet lists (vlist.hip): after the binning of a step, decide whether the
-// lists still hold (no particle moved more than skin / 2 since the build, no
-// incomplete build, no build requested), refresh the frozen copies when they
-// do not, and set the mode the PP kernel reads.  pos: caller order.
-int vlist_prepare(mdf_p3m_ctx *c, const double *pos, int n, hipStream_t s);
+// The binning of a force step (vlist.hip).  Without pair lists: the counting
+// sort.  With them: the positions into the frozen order and the decision
+// whether the lists still hold (no particle moved more than skin / 2 since the
+// build, no incomplete build, no build requested), which sets the mode the
+// binning and the PP kernel read on the device; a build step bins anew and
+// records the build positions.  pos: caller order.
+int bin_step(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s);
 int vlist_alloc(mdf_p3m_ctx *c, double skin);
 void vlist_free(mdf_p3m_ctx *c);
 

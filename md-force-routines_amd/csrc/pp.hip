@@ -90,14 +90,11 @@ struct PPArgs {
     // those within rc and writes every candidate it queued to the list of its
     // i), 2 = evaluate the lists.  A build that cannot list a block (a tile in
     // several passes, a list over vl_cap) sets vl_flags[1], which makes the
-    // next step build again.  spos and cell_start are then the frozen copies
-    // the lists index; the force of frozen slot s goes to current sorted slot
-    // inv_perm[vl_perm[s]].
+    // next step build again.  On a list step spos and cell_start are the
+    // frozen order the lists index (the binning keeps it until the next build).
     int *vl_flags;
-    unsigned short *__restrict__ vl_list;  // vl_cap entries per frozen slot: tile slots of its j
-    int *__restrict__ vl_cnt;               // entries per frozen slot
-    const int *__restrict__ vl_perm;        // frozen slot -> caller index
-    const int *__restrict__ inv_perm;       // caller index -> current sorted slot
+    unsigned short *__restrict__ vl_list;  // vl_cap entries per sorted slot: tile slots of its j
+    int *__restrict__ vl_cnt;               // entries per sorted slot
     int vl_cap;
     T rcl2;                                 // list radius^2: (rc + skin)^2, slightly widened
 };
@@ -745,9 +742,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 }
             }
             if (gl == 0 && have_i) {
-                // list mode: frozen slot gi's force to its current sorted slot
-                const int so = use_list ? P.inv_perm[P.vl_perm[gi]] : gi;
-                double *a = P.acc + 3 * (size_t)so;
+                double *a = P.acc + 3 * (size_t)gi;
                 if (pass == 0) {
                     a[0] = sfx;
                     a[1] = sfy;
@@ -807,13 +802,11 @@ static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, bool l
     PPArgs<R> P;
     P.g = c->g;
     lists = lists && c->vl_skin > 0.0;
-    P.spos = lists ? c->vl_pos : c->spos;
-    P.cell_start = lists ? c->vl_cs : c->cell_start;
+    P.spos = c->spos;
+    P.cell_start = c->cell_start;
     P.vl_flags = lists ? c->vl_flags : nullptr;
     P.vl_list = c->vl_list;
     P.vl_cnt = c->vl_cnt;
-    P.vl_perm = c->vl_perm;
-    P.inv_perm = c->inv_perm;
     P.vl_cap = c->vl_cap;
     const double rl = (c->g.rc + c->vl_skin) * (1.0 + 1e-6);
     P.rcl2 = (R)(rl * rl);

@@ -10,20 +10,23 @@
 // still at least rc apart.  The forces are the reference's up to summation
 // order.
 //
-// A list is indexed by the build's sorted order ("frozen" order) and its tile
-// slots by the build's cell starts, so both are kept until the next build:
-// every step gathers the caller's positions into the frozen order (each taken
-// at the periodic image nearest its build position, so that a wrap by the
-// drift does not move a particle by L in its tile) and records the largest
-// displacement since the build.  The PM stages keep the step's own binning;
-// the PP kernel writes the force of frozen slot s to sorted slot
-// inv_perm[vl_perm[s]].
+// A list is indexed by the build's sorted order and its tile slots by the
+// build's cell starts.  The binning keeps both until the next build ("frozen"
+// order): on a list step it is skipped, and the positions of the step are
+// gathered into the frozen order instead, each at the periodic image nearest
+// its build position (so that a wrap by the drift does not move a particle by
+// L in its tile), while the largest displacement since the build is recorded.
+// Every stage of a list step (PP, assignment, gather) then works in the frozen
+// order; the PM stages take the few ions that left their block's brick through
+// global memory.
 //
-// Per step, after the binning:
-//   k_vl_prep    positions into the frozen order, max displacement^2
-//   k_vl_decide  one thread: build (requested, last build incomplete, or a
-//                displacement over s / 2) or lists; the PP kernel reads the mode
-//   k_vl_refresh on a build step: the frozen copies := this step's binning
+// Per step (bin_step):
+//   k_vl_prep    positions into the frozen order (spos), max displacement^2
+//   k_vl_decide  one block: build (requested, last build incomplete, or a
+//                displacement over s / 2) or lists; the binning and the PP
+//                kernel read the mode
+//   binning      skipped on a list step
+//   k_vl_refresh on a build step: the build positions := the new sorted copy
 #include "mdf_internal.h"
 
 namespace mdf {
@@ -32,9 +35,9 @@ constexpr int kPrepThreads = 256;
 constexpr int kPrepPerThread = 4;  // particles a thread: the block maxima stay few
 
 __global__ __launch_bounds__(kPrepThreads) void k_vl_prep(const double *__restrict__ pos, int n,
-                                                          const int *__restrict__ vl_perm,
+                                                          const int *__restrict__ perm,
                                                           const double4 *__restrict__ vl_ref,
-                                                          double4 *__restrict__ vl_pos, double Lx, double Ly, double Lz,
+                                                          double4 *__restrict__ spos, double Lx, double Ly, double Lz,
                                                           const int *__restrict__ flags, float *__restrict__ bmax) {
     __shared__ float wmax[kPrepThreads / 64];
     if (flags[0]) return;  // a build is requested: the frozen order is replaced anyway
@@ -42,7 +45,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_vl_prep(const double *__restri
     for (int q = 0; q < kPrepPerThread; ++q) {
         const int s = (blockIdx.x * kPrepPerThread + q) * kPrepThreads + threadIdx.x;
         if (s >= n) break;
-        const int i = vl_perm[s];
+        const int i = perm[s];
         const double4 r = vl_ref[s];
         double dx = pos[3 * (size_t)i + 0] - r.x;
         double dy = pos[3 * (size_t)i + 1] - r.y;
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_vl_prep(const double *__restri
         dx -= Lx * rint(dx / Lx);
         dy -= Ly * rint(dy / Ly);
         dz -= Lz * rint(dz / Lz);
-        vl_pos[s] = make_double4(r.x + dx, r.y + dy, r.z + dz, r.w);
+        spos[s] = make_double4(r.x + dx, r.y + dy, r.z + dz, r.w);
         const double d2 = dx * dx + dy * dy + dz * dz;
         d2f = fmaxf(d2f, d2 <= 1e30 ? (float)d2 : 1e30f);  // a non-finite position forces a build
     }
@@ -94,28 +97,19 @@ __global__ __launch_bounds__(1024) void k_vl_decide(int *__restrict__ flags, con
     }
 }
 
-__global__ __launch_bounds__(256) void k_vl_refresh(const double4 *__restrict__ spos, const int *__restrict__ perm,
-                                                    const int *__restrict__ cell_start, int n, int ncell,
-                                                    double4 *__restrict__ vl_pos, double4 *__restrict__ vl_ref,
-                                                    int *__restrict__ vl_perm, int *__restrict__ vl_cs,
-                                                    const int *__restrict__ flags) {
+__global__ __launch_bounds__(256) void k_vl_refresh(const double4 *__restrict__ spos, int n,
+                                                    double4 *__restrict__ vl_ref, const int *__restrict__ flags) {
     if (flags[3] != 1) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) {
-        const double4 p = spos[t];
-        vl_pos[t] = p;
-        vl_ref[t] = p;
-        vl_perm[t] = perm[t];
-    }
-    if (t <= ncell) vl_cs[t] = cell_start[t];
+    if (t < n) vl_ref[t] = spos[t];
 }
 
 __global__ void k_vl_set(int *__restrict__ p, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-int vlist_prepare(mdf_p3m_ctx *c, const double *pos, int n, hipStream_t s) {
-    if (!(c->vl_skin > 0.0) || n <= 0) return MDF_OK;
+int bin_step(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s) {
+    if (!(c->vl_skin > 0.0) || n <= 0) return bin_particles_mode(c, pos, Z, n, nullptr, s);
     if (n != c->vl_n) {  // the frozen order holds the last build's particles only
         c->vl_force_build = 1;
         c->vl_n = n;
@@ -131,8 +125,8 @@ int vlist_prepare(mdf_p3m_ctx *c, const double *pos, int n, hipStream_t s) {
         set_error("pair lists: %d particles exceed the displacement partials", n);
         return MDF_EINVAL;
     }
-    float *bmax = (float *)c->part_red;  // free between the binning and the PP stage
-    hipLaunchKernelGGL(k_vl_prep, dim3(nb), dim3(kPrepThreads), 0, s, pos, n, c->vl_perm, c->vl_ref, c->vl_pos,
+    float *bmax = (float *)c->part_red;  // free until the n(q) stage of the step
+    hipLaunchKernelGGL(k_vl_prep, dim3(nb), dim3(kPrepThreads), 0, s, pos, n, c->perm, c->vl_ref, c->spos,
                        c->g.L[0], c->g.L[1], c->g.L[2], c->vl_flags, bmax);
     MDF_LAUNCHED();
     // build when some particle moved more than skin / 2 (the float threshold
@@ -142,21 +136,20 @@ int vlist_prepare(mdf_p3m_ctx *c, const double *pos, int n, hipStream_t s) {
     if ((double)thr > h * h) thr = nextafterf(thr, 0.0f);
     hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(1024), 0, s, c->vl_flags, bmax, nb, thr);
     MDF_LAUNCHED();
-    const int m = (n > c->ncell + 1 ? n : c->ncell + 1);
-    hipLaunchKernelGGL(k_vl_refresh, dim3((m + 255) / 256), dim3(256), 0, s, c->spos, c->perm, c->cell_start, n,
-                       c->ncell, c->vl_pos, c->vl_ref, c->vl_perm, c->vl_cs, c->vl_flags);
+    MDF_TRY(bin_particles_mode(c, pos, Z, n, c->vl_flags + 3, s));
+    hipLaunchKernelGGL(k_vl_refresh, dim3((n + 255) / 256), dim3(256), 0, s, c->spos, n, c->vl_ref, c->vl_flags);
     MDF_LAUNCHED();
     return MDF_OK;
 }
 
 void vlist_free(mdf_p3m_ctx *c) {
-    void *ptrs[] = {c->vl_list, c->vl_cnt, c->vl_pos, c->vl_ref, c->vl_perm, c->vl_cs, c->vl_flags};
+    void *ptrs[] = {c->vl_list, c->vl_cnt, c->vl_ref, c->vl_flags};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->vl_list = nullptr;
     c->vl_cnt = nullptr;
-    c->vl_pos = c->vl_ref = nullptr;
-    c->vl_perm = c->vl_cs = c->vl_flags = nullptr;
+    c->vl_ref = nullptr;
+    c->vl_flags = nullptr;
     c->vl_skin = 0.0;
     c->vl_cap = 0;
 }
@@ -180,10 +173,7 @@ int vlist_alloc(mdf_p3m_ctx *c, double skin) {
     };
     alloc((void **)&c->vl_list, n * cap * sizeof(unsigned short));
     alloc((void **)&c->vl_cnt, n * sizeof(int));
-    alloc((void **)&c->vl_pos, n * sizeof(double4));
     alloc((void **)&c->vl_ref, n * sizeof(double4));
-    alloc((void **)&c->vl_perm, n * sizeof(int));
-    alloc((void **)&c->vl_cs, ((size_t)c->ncell + 1) * sizeof(int));
     alloc((void **)&c->vl_flags, 8 * sizeof(int));
     if (e != hipSuccess) {
         set_error("pair lists: hipMalloc failed (%s)", hipGetErrorString(e));
