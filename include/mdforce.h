@@ -139,9 +139,11 @@ int mdf_md_run(mdf_p3m_ctx *ctx, double *pos, double *vel, double *acc, const do
 
 /* Stage timing with HIP events recorded on the call's stream, for
  * measurement only.  mode 1: start (nout = event pool size, 0 = default),
- * mode 0: stop, mode 2: synchronise and write per-stage milliseconds
- * out[0..7] and interval counts out[8..15] (nout >= 16) for the stages
- * bin, pp, assign, fft, gather, drift, modes, finalize; resets the totals. */
+ * mode 3: start with events at the PP stage's start and end only (two a
+ * step; only the pp figure is then a stage time), mode 0: stop, mode 2:
+ * synchronise and write per-stage milliseconds out[0..7] and interval counts
+ * out[8..15] (nout >= 16) for the stages bin, pp, assign, fft, gather,
+ * drift, modes, finalize; resets the totals. */
 int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout);
 
 /* Verlet pair lists for the PP stage (no reference counterpart: the reference

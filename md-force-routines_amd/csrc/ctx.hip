@@ -41,6 +41,7 @@ struct Prof {
     std::vector<int> stage;
     size_t used = 0;
     bool overflow = false;
+    bool pp_only = false;  // marks at the PP stage's start and end only (two events a step)
 };
 
 namespace mdf {
@@ -48,6 +49,7 @@ namespace mdf {
 void prof_mark(mdf_p3m_ctx *c, int stage, hipStream_t s) {
     Prof *p = c->prof;
     if (!p) return;
+    if (p->pp_only && stage != kStagePP && stage != kStageAssign) return;
     if (p->used >= p->ev.size()) {
         p->overflow = true;
         return;
@@ -789,7 +791,7 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout) {
         prof_free(ctx);
         return MDF_OK;
     }
-    if (mode == 1) {
+    if (mode == 1 || mode == 3) {
         prof_free(ctx);
         Prof *p = new (std::nothrow) Prof();
         if (!p) {
@@ -809,6 +811,7 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout) {
                 return MDF_EHIP;
             }
         }
+        p->pp_only = mode == 3;
         ctx->prof = p;
         return MDF_OK;
     }
@@ -836,7 +839,7 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout) {
         }
         return MDF_OK;
     }
-    set_error("mdf_p3m_profile: mode must be 0, 1 or 2");
+    set_error("mdf_p3m_profile: mode must be 0, 1, 2 or 3");
     return MDF_EINVAL;
 }
 

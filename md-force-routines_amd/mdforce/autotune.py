@@ -247,3 +247,83 @@ def tune_launch(eng, pos, vel, dt, steps=20, skins=None, min_gain=0.01, log=None
     cfg = eng.launch_config()
     return {"skin": cur["skin"], "requested": dict(cur), "in_use": cfg, "ms_per_step": best,
             "timings": [(w, ms) for w, ms in timings]}
+
+
+def refine_alpha(pos, L, kappa, mesh, rc, ref, precision="mixed", device=None, lo=None, hi=None, iters=8):
+    """alpha of least measured error at (mesh, rc): golden-section search on
+    [lo, hi] (default alpha rc in [2.4, 4.0]; the error is the sum of a PP
+    truncation part falling and a mesh part rising with alpha).  Returns
+    (alpha, error, [(alpha, error), ...])."""
+    from .engine import P3M
+    N = pos.shape[0]
+    lo = 2.4 / rc if lo is None else lo
+    hi = 4.0 / rc if hi is None else hi
+    seen = []
+
+    def err_at(a):
+        e = P3M(N, mesh, L, kappa, a, rc, precision=precision, device=device)
+        try:
+            e.build_green()
+            acc, _ = e.force(pos, None, af=1.0)
+            er = force_error(acc, ref)
+        finally:
+            e.close()
+        seen.append((a, er))
+        return er
+
+    g = (math.sqrt(5.0) - 1.0) / 2.0
+    x1, x2 = hi - g * (hi - lo), lo + g * (hi - lo)
+    f1, f2 = err_at(x1), err_at(x2)
+    for _ in range(iters - 2):
+        if f1 <= f2:
+            hi, x2, f2 = x2, x1, f1
+            x1 = hi - g * (hi - lo)
+            f1 = err_at(x1)
+        else:
+            lo, x1, f1 = x1, x2, f2
+            x2 = lo + g * (hi - lo)
+            f2 = err_at(x2)
+    a, er = min(seen, key=lambda t: t[1])
+    return a, er, seen
+
+
+def tune_measured(pos, vel, L, kappa, dt, budget=1e-5, meshes=(32, 64), margin=0.8, ref=None, device=None,
+                  fallback_meshes=(128,), steps=30, log=None):
+    """The setting of least measured step time among those whose measured
+    error meets margin * budget: tune() gives the cheapest (mesh, rc) of the
+    0.5-wide rc grid by the cost model; rc - 0.25, rc and rc + 0.25 on that
+    mesh then get alpha by refine_alpha, and each that meets the budget is
+    timed with its launch choices tuned (tune_launch: the skin and cell size,
+    PP waves, PM blocks), NVE steps from (pos, vel).  Returns a Choice whose
+    cost is the measured ms per step, with the candidates' timings in tried."""
+    from .engine import P3M
+    N = pos.shape[0]
+    if ref is None:
+        ref = reference_force(P3M, pos, L, kappa, device=device)
+    ch = tune(pos, L, kappa, budget=budget, meshes=meshes, margin=margin, ref=ref, device=device,
+              fallback_meshes=fallback_meshes)
+    rc_max = L / 3.0 * (1.0 - 1e-9)
+    timed = []
+    for rc in (ch.rc - 0.25, ch.rc, ch.rc + 0.25):
+        if rc < 2.0 or rc > rc_max:
+            continue
+        a, er, _ = refine_alpha(pos, L, kappa, ch.mesh, rc, ref, device=device)
+        if er > margin * budget:
+            if log:
+                log(f"[tune_measured] mesh {ch.mesh} rc {rc}: best alpha {a:.4f} error {er:.3e} over the budget")
+            continue
+        e = P3M(N, ch.mesh, L, kappa, a, rc, precision="mixed", device=device)
+        try:
+            e.build_green()
+            tl = tune_launch(e, pos, vel, dt, steps=steps)
+        finally:
+            e.close()
+        timed.append((tl["ms_per_step"], rc, a, er, tl["requested"]))
+        if log:
+            log(f"[tune_measured] mesh {ch.mesh} rc {rc} alpha {a:.4f}: error {er:.3e}, "
+                f"{tl['ms_per_step']:.4f} ms/step with {tl['requested']}")
+    if not timed:
+        return ch
+    ms, rc, a, er, launch = min(timed)
+    return Choice(ch.mesh, a, rc, er, ms, tried=[dict(rc=t[1], alpha=t[2], error=t[3], ms_per_step=t[0],
+                                                      launch=t[4]) for t in timed])

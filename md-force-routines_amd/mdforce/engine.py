@@ -141,8 +141,10 @@ class P3M:
 
     STAGES = ("bin", "pp", "assign", "fft", "gather", "drift", "modes", "finalize")
 
-    def profile_start(self, events=65536):
-        check(self._lib.mdf_p3m_profile(self._ctx, 1, None, int(events)), "mdf_p3m_profile")
+    def profile_start(self, events=65536, pp_only=False):
+        """Stage timing by HIP events; pp_only: two events a step around the PP
+        stage (only the pp figure is then a stage time)."""
+        check(self._lib.mdf_p3m_profile(self._ctx, 3 if pp_only else 1, None, int(events)), "mdf_p3m_profile")
 
     def profile_read(self):
         """Per-stage (total ms, intervals) since the last read; synchronises."""

@@ -130,3 +130,27 @@ def test_tune_launch_picks_a_valid_setting(system):
     assert rel_rms(a, b) <= 2e-6
     e.close()
     ref.close()
+
+
+def test_tune_measured_meets_the_budget_and_times_its_candidates():
+    """alpha refinement at the tuned rc and its neighbours, each candidate
+    timed with its launch choices; the choice meets the budget against an
+    independent tight force."""
+    from mdforce import autotune
+    from mdforce.engine import P3M
+    N = 8000
+    L = O.box_length(N)
+    pos, vel = O.initial_split(31, 32, N, [L, L, L], 0.01)
+    p, v = torch.from_numpy(pos).cuda(), torch.from_numpy(vel).cuda()
+    ref = autotune.reference_force(P3M, p, L, 2.0)
+    ch = autotune.tune_measured(p, v, L, 2.0, 0.05, budget=1e-5, ref=ref, steps=5)
+    print(f"[tune_measured] mesh {ch.mesh} alpha {ch.alpha:.4f} rc {ch.rc} error {ch.error:.2e} "
+          f"{ch.cost:.4f} ms/step; timed {ch.tried}")
+    assert ch.error <= 0.8e-5
+    assert all(t["error"] <= 0.8e-5 and t["ms_per_step"] > 0 for t in ch.tried)
+    assert ch.cost == min(t["ms_per_step"] for t in ch.tried)
+    e = P3M(N, ch.mesh, L, 2.0, ch.alpha, ch.rc, precision="mixed")
+    e.build_green()
+    acc, _ = e.force(p, None, af=1.0)
+    assert autotune.force_error(acc, ref) <= 1e-5
+    e.close()
