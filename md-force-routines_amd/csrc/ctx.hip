@@ -263,7 +263,7 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], md
     {
         int rr = mixed ? build_pair_table(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, true, pair_tab, c->npiece)
                        : build_pair_table(p->kappa, p->alpha, p->rc, kChebDeg, 4e-15, false, pair_tab, c->npiece);
-        if (rr == MDF_OK && mixed) rr = build_pair_table_s(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, s_tab, c->sp);
+        if (rr == MDF_OK && mixed) rr = build_pair_table_s(p->kappa, p->alpha, p->rc, kSDegF, 1e-7, s_tab, c->sp);
         if (rr != MDF_OK) {
             delete c;
             return rr;

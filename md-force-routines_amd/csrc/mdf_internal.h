@@ -80,7 +80,17 @@ struct SPieces {
 constexpr int kPMThreads = 512;
 constexpr int kGatherThreads = 1024;
 constexpr int kChebDeg = 13;        // degree of the PP pair-term Chebyshev pieces (FP64)
-constexpr int kChebDegF = 4;        // degree of the FP32 pieces (mixed precision)
+constexpr int kChebDegF = 4;        // degree of the FP32 r-indexed pieces (mixed precision)
+// degree of the FP32 s-indexed pieces (the pair table in LDS).  Degree 3
+// (238 pieces at kappa 2, alpha 0.43, rc 6.75, against 119) measured 1.450
+// against 1.430 ms/step at N = 1e6: one LDS read and two FMAs fewer a pair,
+// but the 3 KB larger table cost the two 12-wave workgroups a CU at rc 7
+constexpr int kSDegF = 4;
+// LDS stride of an s-indexed piece in (phi, F) coefficient pairs: an odd
+// count, so that a piece spans 2 (mod 4) dwords and the pieces of a wave's
+// lanes spread over the 64 LDS banks (at 8 dwords, pieces 8 apart share banks:
+// degree 3 unpadded ran 1.74 against 1.45 ms/step padded)
+constexpr int kSStride = (kSDegF + 1) | 1;
 constexpr int kMaxPartials = 1 << 20;
 
 // ctx->stats (device ints): [0] PP tiles needing more than one LDS pass,
@@ -120,7 +130,7 @@ struct mdf_p3m_ctx {
     double2 *cheb;      // PP pair-term tables, npiece * (kChebDeg + 1) (g1, g2) pairs
     int npiece;
     float2 *chebf;      // FP32 tables of the mixed mode, npiece * (kChebDegF + 1)
-    float2 *stab;       // FP32 s-indexed pair table of the mixed mode, sp.npiece * (kChebDegF + 1)
+    float2 *stab;       // FP32 s-indexed pair table of the mixed mode, sp.npiece * (kSDegF + 1)
     mdf::SPieces sp;
 
     // mesh

@@ -135,9 +135,8 @@ __device__ __forceinline__ void cheb2(C c, T t, T &g1, T &g2) {
 }
 
 // Horner sums of both power series of one piece at t in [-1, 1] (FP32 tables).
-template <class T, class C>
+template <class T, int D = ChebDeg<T>::v, class C>
 __device__ __forceinline__ void horner2(C c, T t, T &g1, T &g2) {
-    constexpr int D = ChebDeg<T>::v;
     T a = c[D].x, b = c[D].y;
 #pragma unroll
     for (int k = D - 1; k >= 0; --k) {
@@ -229,7 +228,8 @@ __device__ __forceinline__ void pp_pair_masked(const PPArgs<T> &P, const typenam
 template <bool LIST = false>
 __device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, const float2 *__restrict__ stab, bool ok, float dx,
                                           float dy, float dz, float &fx, float &fy, float &fz, float &u) {
-    constexpr int D = kChebDegF;
+    constexpr int D = kChebDegF;  // the r-indexed table of the close-pair path
+    constexpr int DS = kSDegF;    // the s-indexed table in LDS
     const float s2 = dx * dx + dy * dy + dz * dz;
     const bool in = LIST ? s2 < P.rc2 : ok && s2 < P.rc2 && s2 > 0.0f;
     const unsigned b = __float_as_uint(s2);
@@ -238,9 +238,9 @@ __device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, const float2 *
     const unsigned p = min((b >> P.s_shift) - P.s_base, (unsigned)P.ntab - 1u);
     const float t = __uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u) - P.s_tc;
     using LdsF2 = __attribute__((address_space(3))) const float2;
-    LdsF2 *c = (LdsF2 *)stab + __umul24(p, D + 1);
+    LdsF2 *c = (LdsF2 *)stab + __umul24(p, kSStride);
     float phi, F;
-    horner2<float>(c, t, phi, F);
+    horner2<float, DS>(c, t, phi, F);
     if (LIST ? s2 < P.s_lo : in && s2 < P.s_lo) {
         const float ir = rsqrt_t<float>(s2);
         const float r = s2 * ir;
@@ -305,8 +305,10 @@ __device__ __forceinline__ int list_padded(int cnt) { return (cnt + 2 + 2 * kGL 
 template <class R, int NW, bool LISTS>
 __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsigned char *ldsraw) {
     using R2 = typename Vec2<R>::t;
-    constexpr int D = ChebDeg<R>::v;
     constexpr bool kRel = sizeof(R) == 4;  // tile relative to the home block centre
+    // degree of the table staged in LDS: FP32 the s-indexed pieces, FP64 the
+    // r-indexed Chebyshev pieces
+    constexpr int D = kRel ? kSDegF : ChebDeg<R>::v;
     // FP32 tile: hi and lo as two float4 (32 B a particle), or packed into a
     // float4 (hi x, y, z, lo x) and a float2 (lo y, z) (24 B) for the 12-wave
     // kernel, where the smaller tile is what lets two workgroups share a CU;
@@ -322,7 +324,8 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     float4 *tl4 = th4 + T;                                 // mixed, unpacked: lo (x, y, z)
     float2 *tl2 = (float2 *)(th4 + T);                     // mixed, packed: lo (y, z)
     R2 *tab = kRel ? (kPack ? (R2 *)(tl2 + T) : (R2 *)(tl4 + T)) : (R2 *)(tz + T);  // npiece * (D + 1)
-    int *cum = (int *)(tab + P.ntab * (D + 1));            // 217 prefix offsets of the tile sub-cells
+    constexpr int TS = kRel ? kSStride : D + 1;  // LDS stride of a table piece
+    int *cum = (int *)(tab + P.ntab * TS);                 // 217 prefix offsets of the tile sub-cells
     int *queue = cum + 224;                                // NW * kGroups * kQCap
     __shared__ int tstart[216];  // sorted-array start of each tile sub-cell
     __shared__ int2 rng[8][5];   // per home sub-cell: LDS range of each of its 5 stencil planes
@@ -341,7 +344,10 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     const int by = (bid / g.nblk[0]) % g.nblk[1];
     const int bz = bid / (g.nblk[0] * g.nblk[1]);
 
-    for (int t = tid; t < P.ntab * (D + 1); t += blockDim.x) tab[t] = P.cheb[t];
+    for (int t = tid; t < P.ntab * (D + 1); t += blockDim.x) {
+        const int pc = t / (D + 1), k = t - pc * (D + 1);
+        tab[pc * TS + k] = P.cheb[t];
+    }
     if (LISTS && tid == 0) {
         if constexpr (kRel) {
             th4[kFar] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
@@ -793,7 +799,7 @@ size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
     // tile: (x, y, z) in FP64 (24 B a particle); FP32 hi and lo float4 (32 B),
     // packed to 24 B in the 12-wave kernel
     const size_t es = mixed ? 4 : 8;
-    const size_t ncoef = (size_t)npiece * ((mixed ? kChebDegF : kChebDeg) + 1);
+    const size_t ncoef = (size_t)npiece * (mixed ? kSStride : kChebDeg + 1);
     return ((mixed && nw != 12) ? 32 : 24) * (size_t)tile_max + 2 * es * ncoef + 4 * (size_t)(224 + nw * kGroups * kQCap);
 }
 

@@ -127,26 +127,19 @@ def test_mixed_n1e5_error_vs_fp64(P3M):
     assert abs(u32[0] - u64[0]) <= ENERGY_TOL * abs(u64[0])
 
 
-@pytest.mark.parametrize("rc, waves", [(5.5, 8), (7.0, 12), (8.5, 16)])
+@pytest.mark.parametrize("rc, waves", [(5.5, 4), (5.5, 8), (7.0, 12), (8.5, 16)])
 def test_mixed_pp_wave_counts_vs_fp64(P3M, rc, waves):
     """Each FP32 PP kernel variant against the FP64 GPU path at unit density:
-    8 waves, 12 waves with the packed 24-byte tile, and 16 waves with tiles
-    over the LDS capacity (several passes).  The wave count follows from the
-    tile size, which depends on the cell geometry (floor(L/rc)); N is taken as
-    the first of a few sizes whose context picks the wanted count."""
+    4 and 8 waves, 12 waves with the packed 24-byte tile, and 16 waves with
+    tiles over the LDS capacity (several passes), the wave count set with
+    mdf_p3m_set_launch."""
     kappa = 2.0
     alpha = 2.8 / rc
-    chosen = None
-    for N in (60000, 50000, 80000, 40000, 100000, 30000, 120000, 150000):
-        L = O.box_length(N)
-        e = P3M(N, 64, L, kappa, alpha, rc, precision="mixed")
-        if e.config()["pp_waves"] == waves:
-            chosen = (N, L, e)
-            break
-        e.close()
-    if chosen is None:
-        pytest.fail(f"no size picks {waves} waves at rc {rc}")
-    N, L, e_mixed = chosen
+    N = 60000
+    L = O.box_length(N)
+    e_mixed = P3M(N, 64, L, kappa, alpha, rc, precision="mixed")
+    e_mixed.set_launch(waves)
+    assert e_mixed.config()["pp_waves"] == waves
     rng = np.random.default_rng(11)
     pos = dev(rng.random((N, 3)) * L)
     outs = {}
