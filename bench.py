@@ -107,8 +107,8 @@ def pmc_traffic(stage, form, N, M, rc, alpha):
     try:
         for line in open(PMC_FILE):
             if line.startswith("traffic k_pp<float"):
-                t = line.split()
-                kv = dict(zip(t[2::2], map(float, t[3::2])))
+                t = line.split(">", 1)[1].split()  # key value pairs after the kernel name
+                kv = dict(zip(t[0::2], map(float, t[1::2])))
                 # the PP kernel's traffic depends on N, the mesh and the cells
                 # (rc), not on alpha
                 if (kv.get("N"), kv.get("M"), kv.get("rc")) != (N, M, rc):
