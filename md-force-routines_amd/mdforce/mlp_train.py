@@ -208,3 +208,32 @@ class LearnedForce:
 
     def md_run(self, pos, vel, acc, nsteps, dt, *args, **kw):
         return self.engine.md_run(pos, vel, acc, nsteps, dt, *args, **kw)
+
+
+def radial_design(net, eng, pos, centers, width, chunk=4_000_000):
+    """Design matrix of a radial pair correction f(r) = sum_k c_k b_k(r) with
+    Gaussian b_k (centres ``centers``, width ``width``): G[i, k] = sum_j
+    b_k(r_ij) u_ij, the force on ion i (caller order) of basis k, over the pairs
+    r_ij < eng.rc from the network's pair list (u_ij the unit vector from j to
+    i).  Returns (G of shape (N, K, 3) float64, pair count)."""
+    d = net.pairs(eng, pos)
+    P, perm = d["npairs"], d["perm"]
+    K = centers.numel()
+    G = torch.zeros((pos.shape[0], K, 3), dtype=torch.float64, device=pos.device)
+    for s in range(0, P, chunk):
+        i = perm[d["i"][s:s + chunk]]
+        r = d["r"][s:s + chunk].double()
+        u = d["u"][s:s + chunk].double()
+        b = torch.exp(-0.5 * ((r[:, None] - centers[None, :]) / width) ** 2)
+        G.index_add_(0, i, b[:, :, None] * u[:, None, :])
+    return G, P
+
+
+def fit_radial(G, residual, ridge=1e-10):
+    """Least-squares coefficients c of sum_k c_k G[:, k] ~ residual (N, 3), with
+    a ridge of ``ridge`` times the mean diagonal of the normal matrix."""
+    A = torch.einsum("nka,nla->kl", G, G)
+    b = torch.einsum("nka,na->k", G, residual)
+    K = A.shape[0]
+    lam = ridge * float(torch.trace(A)) / K
+    return torch.linalg.solve(A + lam * torch.eye(K, dtype=A.dtype, device=A.device), b)
