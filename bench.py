@@ -34,6 +34,7 @@ import json
 import math
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -73,7 +74,7 @@ def parse():
                     help="Verlet pair-list skin of the PP stage with --no-autotune (0 = off); the cells of "
                          "rc + skin set the skin actually used (the largest they allow).  The autotuner picks "
                          "the skin, PP waves and PM block shapes by timing")
-    ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "radial", "drift", "sqw"),
+    ap.add_argument("--workload", choices=("n1e6", "n8e6", "replicas", "mlp", "radial", "drift", "sqw", "cpu"),
                     default="n1e6",
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
@@ -88,6 +89,9 @@ def parse():
                     help="alpha and rc from the autotuner (FP32-mixed, budget 1e-5, the given mesh, tuned on the "
                          "equilibrated liquid); --no-autotune uses --alpha/--rc")
     ap.add_argument("--graph-steps", type=int, default=50, help="0 = no graph (eager launches)")
+    ap.add_argument("--cpu-configs", default="cfg1,cfg2",
+                    help="cpu workload: comma-separated cfg1 (= cfg1-1t and cfg1-all: one thread, all threads "
+                         "and the GPU), cfg2")
     ap.add_argument("--streams", type=int, default=16, help="worker streams of the replica graph")
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both",
                     help="forms to time; value is the chosen form's unless only plain runs")
@@ -129,6 +133,13 @@ def host_cpu():
                 model = line.split(":", 1)[1].strip()
                 break
     except OSError:
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
         pass
     try:
         avail = len(os.sched_getaffinity(0))
@@ -458,6 +469,170 @@ def run_radial(a):
     print(json.dumps(out), flush=True)
 
 
+def run_cpu(a):
+    """The CPU baselines of the measurement plan, on this host, beside the GPU.
+
+    cfg1 in full: N = 1e4, Gamma 100, kappa 2, 64^3, alpha 0.85, rc 6.2,
+    dt 0.1, the seeded start, 500 steps with the velocity rescale of
+    thermostat_p3m_ord6.py:5-25 (first branch), then 1000 NVE steps with n(q,t)
+    for the 20 smallest q along x; through the C restatement on one thread
+    (serial build) and on all threads (OpenMP build), and the plain FP64 GPU
+    path on the same protocol.  cfg2: 5 NVE steps of the N = 1e6 liquid
+    (128^3, alpha 0.4329, rc 6.75) on one thread and on all threads.
+    Throughput counts the NVE steps only.  One JSON line with every row."""
+    import torch
+
+    from oracle import c_oracle
+
+    c_oracle.build(ref=False)
+    host = host_cpu()
+    nthr = c_oracle.lib_omp(0)[1]
+    dev = torch.device("cuda", 0)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    rows = []
+    want = set(a.cpu_configs.split(","))
+    if want & {"cfg1", "cfg1-1t", "cfg1-all"}:
+        rows += cpu_cfg1_rows(want, nthr, dev, log)
+    if "cfg2" in want:
+        rows += cpu_cfg2_rows(a, nthr, dev, log)
+    main = rows[1] if len(rows) > 1 else rows[0]
+    out = {
+        "metric": "CPU baselines of the P3M NVE step (C restatement of the reference algorithm)",
+        "value": main["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": 5 if want == {"cfg2"} else 1000, "warmup": 0 if want == {"cfg2"} else 500,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "cfg1 in full and cfg2 for 5 steps on the host CPU, beside the GPU",
+                   "host": host, "rows": rows, "parallelism": "host threads"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_cfg1_rows(want, nthr, dev, log):
+    """cfg1 in full on one thread (cfg1-1t), on all threads and the GPU (cfg1-all)."""
+    import numpy as np
+    import torch
+
+    from mdforce import autotune
+    from mdforce.engine import P3M
+    from mdforce.initialize_pos_vel import initial_split
+    from oracle import c_oracle
+    rows = []
+    N, kappa, gamma, M, alpha, rc, dt = 10000, 2.0, 100.0, 64, 0.85, 6.2, 0.1
+    T = 1.0 / gamma
+    L = (4.0 * math.pi * N / 3.0) ** (1.0 / 3.0)
+    pos0, vel0 = initial_split(1, 2, N, [L, L, L], T)
+    G_k, _ = c_oracle.gf_opt(kappa, alpha, 6, 3, M, L)
+    q = 2.0 * math.pi / L * np.arange(1, 21, dtype=np.float64)
+    Z = np.ones(N)
+
+    def cpu_cfg1(threads, label):
+        p, v = pos0.copy(), vel0.copy()
+        _, acc, _, _, _ = c_oracle.force_pot(p, Z, L, kappa, alpha, rc, G_k, M)
+        t0 = time.perf_counter()
+        for it in range(500):
+            c_oracle.md_steps(p, v, acc, Z, 1, dt, L, kappa, alpha, rc, G_k, M, threads=threads)
+            Tn = float(np.sum(v ** 2)) / N  # thermostat_p3m_ord6.py:12-17
+            v *= math.sqrt(T / Tn)
+            if it % 100 == 99:
+                log(f"[cpu] cfg1 {label}: equilibration step {it + 1}")
+        t_eq = time.perf_counter() - t0
+        en_all, t_nve = [], 0.0
+        for chunk in range(10):
+            t0 = time.perf_counter()
+            en, _ = c_oracle.md_steps(p, v, acc, Z, 100, dt, L, kappa, alpha, rc, G_k, M, q=q, threads=threads)
+            t_nve += time.perf_counter() - t0
+            en_all.append(en)
+            log(f"[cpu] cfg1 {label}: NVE step {100 * (chunk + 1)} ({t_nve:.1f} s)")
+        en = np.concatenate(en_all)
+        drift = float(np.max(np.abs(en[:, 0] - en[0, 0])) / (N * T))
+        return p, {"config": "cfg1 N=1e4 (500 eq + 1000 NVE)", "form": f"CPU C restatement, {label}",
+                   "particle_steps_per_s": N * 1000 / t_nve, "nve_s": t_nve, "eq_s": t_eq,
+                   "drift_per_NT": drift}
+
+    if want & {"cfg1", "cfg1-1t"}:
+        _, r = cpu_cfg1(None, "1 thread (serial build, -O2)")
+        r["threads"] = 1
+        rows.append(r)
+    if not want & {"cfg1", "cfg1-all"}:
+        return rows
+    _, r = cpu_cfg1(0, f"{nthr} threads (OpenMP build, -O3)")
+    r["threads"] = nthr
+    rows.append(r)
+    # the GPU plain form on the same protocol
+    e = P3M(N, M, L, kappa, alpha, rc, device=dev)
+    e.set_green(torch.from_numpy(G_k).to(dev))
+    p = torch.from_numpy(pos0).to(dev)
+    v = torch.from_numpy(vel0).to(dev)
+    acc, _ = e.force(p)
+    acc = acc.clone()
+    for it in range(500):
+        e.md_run(p, v, acc, 1, dt)
+        ks = (v * v).sum().reshape(1)
+        e.thermostat(v, ks, T, it)
+    qd = torch.from_numpy(q).to(dev)
+    e.md_run(p, v, acc, 5, dt, q=qd)
+    torch.cuda.synchronize()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    en, _ = e.md_run(p, v, acc, 1000, dt, q=qd)
+    s1.record()
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1)
+    enh = en.cpu().numpy()
+    ref = autotune.reference_force(P3M, p, L, kappa, device=dev)
+    a1, _ = e.force(p, None, af=1.0)
+    rows.append({"config": "cfg1 N=1e4 (500 eq + 1000 NVE)", "form": "plain FP64 GPU (MI355X)",
+                 "particle_steps_per_s": N * 1000 / (ms / 1e3), "nve_s": ms / 1e3,
+                 "drift_per_NT": float(np.max(np.abs(enh[:, 0] - enh[0, 0])) / (N * T)),
+                 "force_error_rel_meanF": autotune.force_error(a1, ref)})
+    e.close()
+    log(f"[cpu] cfg1 GPU plain: {ms:.1f} ms for 1000 steps")
+
+    return rows
+
+
+def cpu_cfg2_rows(a, nthr, dev, log):
+    """5 NVE steps of the N = 1e6 liquid on one thread and on all threads."""
+    import numpy as np
+    import torch
+
+    from mdforce import mlp_train
+    from mdforce.engine import P3M
+    from mdforce.initialize_pos_vel import initial_split
+    from oracle import c_oracle
+    rows = []
+    kappa, T, dt = 2.0, 0.01, 0.1
+    N, M, alpha, rc = 1_000_000, 128, 0.43288602577760327, 6.75
+    L = (4.0 * math.pi * N / 3.0) ** (1.0 / 3.0)
+    pos0, vel0 = initial_split(1, 2, N, [L, L, L], T)
+    p = torch.from_numpy(pos0).to(dev)
+    v = torch.from_numpy(vel0).to(dev)
+    eq = P3M(N, M, L, kappa, alpha, rc, precision="mixed", device=dev)
+    eq.build_green()
+    mlp_train.equilibrate(eq, p, v, T, a.eq_steps, dt, vcap=4.0, soft_steps=100)
+    eq.close()
+    e = P3M(N, M, L, kappa, alpha, rc, device=dev)
+    G_full, _ = e.build_green(6, 3, want_full=True)
+    acc0 = e.force(p)[0].cpu().numpy()
+    e.close()
+    Gh = G_full.cpu().numpy()
+    ph, vh = p.cpu().numpy(), v.cpu().numpy()
+    qh = 2.0 * math.pi / L * np.arange(1, 21, dtype=np.float64)
+    for threads, label in ((None, "1 thread (serial build, -O2)"), (0, f"{nthr} threads (OpenMP build, -O3)")):
+        pp, vv, aa = ph.copy(), vh.copy(), acc0.copy()
+        t0 = time.perf_counter()
+        for k in range(5):
+            c_oracle.md_steps(pp, vv, aa, np.ones(N), 1, dt, L, kappa, alpha, rc, Gh, M, q=qh, threads=threads)
+            log(f"[cpu] cfg2 {label}: step {k + 1} ({time.perf_counter() - t0:.1f} s)")
+        tc = time.perf_counter() - t0
+        rows.append({"config": "cfg2 N=1e6 liquid, 128^3, alpha 0.4329, rc 6.75 (5 NVE steps)",
+                     "form": f"CPU C restatement, {label}", "threads": 1 if threads is None else nthr,
+                     "particle_steps_per_s": N * 5 / tc, "nve_s": tc})
+    return rows
+
+
 def run_drift(a):
     """Energy conservation and force error along a trajectory, both forms.
 
@@ -643,6 +818,9 @@ def main():
         return
     if a.workload == "radial":
         run_radial(a)
+        return
+    if a.workload == "cpu":
+        run_cpu(a)
         return
     if a.workload == "drift":
         run_drift(a)
