@@ -292,10 +292,12 @@ def tune_measured(pos, vel, L, kappa, dt, budget=1e-5, meshes=(32, 64), margin=0
     """The setting of least measured step time among those whose measured
     error meets margin * budget: tune() gives the cheapest (mesh, rc) of the
     0.5-wide rc grid by the cost model; rc - 0.25, rc and rc + 0.25 on that
-    mesh then get alpha by refine_alpha, and each that meets the budget is
-    timed with its launch choices tuned (tune_launch: the skin and cell size,
-    PP waves, PM blocks), NVE steps from (pos, vel).  Returns a Choice whose
-    cost is the measured ms per step, with the candidates' timings in tried."""
+    mesh then get alpha by refine_alpha (and, while the smallest of them meets
+    the budget, rc - 0.5, rc - 0.75, ... until one does not), and each that
+    meets the budget is timed with its launch choices tuned (tune_launch: the
+    skin and cell size, PP waves, PM blocks), NVE steps from (pos, vel).
+    Returns a Choice whose cost is the measured ms per step, with the
+    candidates' timings in tried."""
     from .engine import P3M
     N = pos.shape[0]
     if ref is None:
@@ -304,7 +306,9 @@ def tune_measured(pos, vel, L, kappa, dt, budget=1e-5, meshes=(32, 64), margin=0
               fallback_meshes=fallback_meshes)
     rc_max = L / 3.0 * (1.0 - 1e-9)
     timed = []
-    for rc in (ch.rc - 0.25, ch.rc, ch.rc + 0.25):
+    rcs = [ch.rc + 0.25, ch.rc, ch.rc - 0.25]
+    while rcs:
+        rc = rcs.pop()
         if rc < 2.0 or rc > rc_max:
             continue
         a, er, _ = refine_alpha(pos, L, kappa, ch.mesh, rc, ref, device=device)
@@ -312,6 +316,8 @@ def tune_measured(pos, vel, L, kappa, dt, budget=1e-5, meshes=(32, 64), margin=0
             if log:
                 log(f"[tune_measured] mesh {ch.mesh} rc {rc}: best alpha {a:.4f} error {er:.3e} over the budget")
             continue
+        if rc <= ch.rc - 0.25 and rc - 0.25 >= 2.0:
+            rcs.append(rc - 0.25)  # the smallest so far meets the budget: try below it
         e = P3M(N, ch.mesh, L, kappa, a, rc, precision="mixed", device=device)
         try:
             e.build_green()
