@@ -130,3 +130,73 @@ def test_skin_too_large_for_the_box_is_refused(P3M, system):
     torch.cuda.synchronize()
     assert torch.isfinite(a).all()
     e.close()
+
+
+def _dense_blob(S, factor, radius, seed):
+    """The lattice system with the ions inside a sphere pulled towards its
+    centre by ``factor`` (local density x factor^-3)."""
+    pos = S["pos"].copy()
+    c = np.full(3, S["L"] / 2)
+    d = pos - c
+    r = np.linalg.norm(d, axis=1)
+    inside = r < radius
+    pos[inside] = c + d[inside] * factor
+    rng = np.random.RandomState(seed)
+    pos[inside] += rng.uniform(-0.05, 0.05, (inside.sum(), 3))
+    return np.mod(pos, S["L"]), int(inside.sum())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_list_overflow_builds_every_step_with_correct_forces(P3M, system, precision):
+    """Ions whose neighbour count exceeds the list capacity (1.4x the mean +
+    64, rounded up to 64): the build marks itself incomplete, every step builds again and the
+    forces stay those of the list-free path."""
+    S = system
+    P, N = S["P"], S["N"]
+    pos0, n_in = _dense_blob(S, 0.6, 9.0, 21)  # density x 4.6 in the blob (radius 5.4)
+    e = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision=precision)
+    e.set_green(dev(P.G_k))
+    e.set_skin(0.5)
+    ref = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision=precision)
+    ref.set_green(dev(P.G_k))
+    pos, vel = dev(pos0), dev(S["vel"]) * 0.1
+    acc, _ = e.force(pos)
+    acc = acc.clone()
+    e.md_run(pos, vel, acc, 4, 0.002)
+    torch.cuda.synchronize()
+    st = e.list_stats()
+    a_ref, _ = ref.force(pos)
+    torch.cuda.synchronize()
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    err = rel_rms(acc.cpu().numpy(), a_ref.cpu().numpy())
+    print(f"[lists] {precision}: {n_in} ions in the blob, cap {st['list_cap']}: {st}, err {err:.2e}")
+    assert st["list_steps"] == 0 and st["builds"] >= 5, st
+    assert err <= tol
+
+
+def test_list_steps_after_pm_only_and_charge_calls_rebuild(P3M, system):
+    """A PM-only call between two md_run calls bins the positions outside a
+    step and so replaces the lists' frozen order; md_run builds on entry and
+    then steps on its lists, with the list-free forces, under non-unit
+    charges (the frozen order carries them from the build)."""
+    S = system
+    P, N = S["P"], S["N"]
+    e = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision="fp64")
+    e.set_green(dev(P.G_k))
+    e.set_skin(0.4)
+    ref = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision="fp64")
+    ref.set_green(dev(P.G_k))
+    pos, vel = dev(S["pos"]), dev(S["vel"])
+    Z = dev(np.random.RandomState(4).uniform(0.5, 1.5, N))
+    acc, _ = e.force(pos, Z)
+    acc = acc.clone()
+    e.md_run(pos, vel, acc, 3, 0.02, Z=Z)
+    b0 = e.list_stats()["builds"]
+    e.pm(pos, Z)  # bins outside a step
+    e.md_run(pos, vel, acc, 3, 0.02, Z=Z)
+    torch.cuda.synchronize()
+    st = e.list_stats()
+    a_ref, _ = ref.force(pos, Z)
+    torch.cuda.synchronize()
+    assert st["builds"] > b0 and st["list_steps"] >= 2, st
+    assert rel_rms(acc.cpu().numpy(), a_ref.cpu().numpy()) <= 1e-12
