@@ -79,6 +79,8 @@ def parse():
                     help="n1e6: one N = 1e6 system (default); replicas: 64 seeded replicas, N 2,000-50,000, "
                          "autotuned, stepped by one graph launch per --graph-steps steps")
     ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--replica-meshes", type=lambda t: tuple(int(x) for x in t.split(",")), default=(32, 64, 128),
+                    help="mesh sizes the replica autotuner considers (comma separated)")
     ap.add_argument("--mlp-epochs", type=int, default=20)
     ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--eq-steps", type=int, default=300)
@@ -180,8 +182,13 @@ def run_replicas(a):
     (100 steps at dt/10, then --eq-steps steps with a velocity rescale every
     10 steps and a per-ion speed clip, on a provisional setting: 32^3 or 64^3
     mesh, rc = min(5, L/3), alpha = 3.2/rc), and its (mesh, alpha, rc) is
-    then taken from the autotuner on the equilibrated liquid, where an RMS
+    then taken from the autotuner (meshes --replica-meshes, default 32^3,
+    64^3 and 128^3, in order of modelled cost) on the equilibrated liquid, where an RMS
     force error <= 1e-5 of the mean force is the harder bar (FP32-mixed).
+    With --autotune (default) the pair-list skins are then chosen by
+    measurement (ReplicaBatch.tune_skins: per replica the fastest alone of no
+    lists and skins 0.25, 0.5 and 1, then kept if the ensemble with them
+    beats the ensemble without lists).
     Equilibration and tuning are outside the timed region.  Timed: `steps`
     NVE steps of all replicas from their equilibrated states with n(q, t)
     (20 q per axis) recorded on device, one graph launch per --graph-steps
@@ -216,7 +223,8 @@ def run_replicas(a):
         eq.build_green()
         mlp_train.equilibrate(eq, p, v, 1.0 / gammas[r], a.eq_steps, a.dt, vcap=4.0, soft_steps=100)
         eq.close()
-        ch = autotune.tune(p, L, float(kappas[r]), budget=1e-5, precision="mixed", device=dev)
+        ch = autotune.tune(p, L, float(kappas[r]), budget=1e-5, meshes=a.replica_meshes, precision="mixed",
+                           device=dev)
         errs.append(ch.error)
         systems.append(dict(n=N, L=L, kappa=float(kappas[r]), mesh=ch.mesh, alpha=ch.alpha, rc=ch.rc,
                             precision="mixed"))
@@ -230,6 +238,12 @@ def run_replicas(a):
         acc, _ = rep.force(P[r])
         A.append(acc.clone())
     nq = a.nq
+    skins = [0.0] * R
+    if a.autotune:
+        t_sk = time.perf_counter()
+        skins = b.tune_skins(P, V, A, a.dt, nq=nq,
+                             log=lambda m: print(f"[skin] {m}", file=sys.stderr, flush=True))
+        t_tune += time.perf_counter() - t_sk
     K = min(a.graph_steps, a.steps)
     if a.warmup:
         b.run(P, V, A, a.warmup, a.dt, nq=nq, graph_steps=min(K, a.warmup))
@@ -253,6 +267,7 @@ def run_replicas(a):
         e = energies[r].cpu().numpy()
         drift.append(float(np.max(np.abs(e[:, 0] - e[0, 0])) / (Ns[r] / gammas[r])))
     nq_ok = all(bool(torch.isfinite(torch.view_as_real(t)).all()) for t in nqt)
+    lstats = [rep.list_stats() for rep in b.replicas]
     out = {
         "metric": METRIC + " (replica ensemble)",
         "value": value,
@@ -277,6 +292,10 @@ def run_replicas(a):
             "meshes": sorted(set(s["mesh"] for s in systems)),
             "rc_range": [min(s["rc"] for s in systems), max(s["rc"] for s in systems)],
             "tuned_on": "equilibrated liquid", "eq_and_tune_s": t_tune, "wall_s": wall,
+            "pair_lists": {"replicas_with_lists": sum(1 for x in skins if x > 0),
+                           "skin_range": [min(skins), max(skins)],
+                           "builds": sum(st["builds"] for st in lstats),
+                           "list_steps": sum(st["list_steps"] for st in lstats)},
             "energy_drift_per_NT_max": max(drift), "nqt_finite": nq_ok,
             "parallelism": "single GPU, replicas overlapped on worker streams inside one hipGraph",
         },

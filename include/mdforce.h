@@ -149,16 +149,23 @@ int mdf_p3m_profile(mdf_p3m_ctx *ctx, int mode, double *out, int nout);
 /* Verlet pair lists for the PP stage (no reference counterpart: the reference
  * rebuilds its cell list and tests every candidate pair on every call,
  * yukawa_pp.py:44-112).  skin > 0: the context's cells become
- * L / floor(L / (rc + skin)) wide (rc + skin <= L/3 per axis), and a step
- * tests candidates only when some particle has moved more than skin / 2
- * since the last test ("build"), writing every pair within rc + skin to
- * per-particle lists; the other steps evaluate the listed pairs with the exact
- * cutoff r < rc.  Forces and energies are those of skin = 0 up to summation
- * order.  mdf_p3m_force, mdf_pp_force and the first step of each mdf_md_run
- * always build.  skin = 0 turns the lists off.  The context is rebuilt in
- * place (the Green function and an attached network carry over; stage
- * profiling stops).  Replica batches step without lists. */
+ * w = L / floor(L / (rc + skin)) wide (rc + skin <= L/3 per axis), and the
+ * lists take the list skin s = w - rc (the largest the cells allow; at most
+ * the cap of mdf_p3m_set_list_skin): a step tests candidates only when some
+ * particle has moved more than s / 2 since the last test ("build"), writing
+ * every pair within rc + s to per-particle lists; the other steps evaluate
+ * the listed pairs with the exact cutoff r < rc.  Forces and energies are
+ * those of skin = 0 up to summation order.  mdf_p3m_force, mdf_pp_force, the
+ * first step of each mdf_md_run and of each mdf_batch_run always build.
+ * skin = 0 turns the lists off.  The context is rebuilt in place (the Green
+ * function and an attached network carry over; stage profiling stops). */
 int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin);
+/* Cap on the list skin s (no reference counterpart): s = min(w - rc,
+ * list_skin), 0 = no cap (the default).  A smaller skin lists fewer pairs and
+ * builds more often.  Keeps the cells; reallocates the lists when s changes.
+ * effective (host pointer, may be null): the list skin now in use (0 without
+ * lists). */
+int mdf_p3m_set_list_skin(mdf_p3m_ctx *ctx, double list_skin, double *effective);
 /* host pointer, 4 ints: list builds and list-evaluation steps since the
  * skin was set, list entries per particle, reference cells along x. */
 int mdf_p3m_list_stats(mdf_p3m_ctx *ctx, int *out4);

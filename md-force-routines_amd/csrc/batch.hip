@@ -44,6 +44,7 @@ struct mdf_batch {
     double dt = 0, af = 0, uf = 0, kf = 0;
     std::vector<double *> cap_pos, cap_vel, cap_acc, cap_e, cap_q;
     std::vector<int> cap_n;
+    std::vector<int> cap_gen;  // the replicas' rebuild generations at capture
     // device ring buffers and flush table
     double *ring_e = nullptr;  // nrep * K * 4
     double *ring_q = nullptr;  // nrep * K * nq * 6
@@ -166,6 +167,7 @@ static int seg_prepare(mdf_batch *b) {
         size_t lds = 0;
         size_t off = 0;
         int total = 0;
+        int variant = 0;
     };
     std::vector<Group> groups;
     std::vector<std::vector<unsigned char>> args(nst);
@@ -192,20 +194,26 @@ static int seg_prepare(mdf_batch *b) {
             const int rc = st.build(b->ctx[r], io, args[i].data() + st.arg_size * r, dims[i][r], lds[i][r]);
             if (rc != MDF_OK) return rc;
         }
-        // groups of similar LDS, in order of LDS
+        // groups of one kernel variant (dims.w) and similar LDS, in order of
+        // variant and LDS
         std::vector<int> order(R);
         for (int r = 0; r < R; ++r) order[r] = r;
-        std::sort(order.begin(), order.end(), [&](int a, int c) { return lds[i][a] < lds[i][c]; });
+        std::sort(order.begin(), order.end(), [&](int a, int c) {
+            return dims[i][a].w != dims[i][c].w ? dims[i][a].w < dims[i][c].w : lds[i][a] < lds[i][c];
+        });
         Group g;
         g.stage = i;
         size_t gmin = 0;
         for (int r : order) {
-            if (!g.reps.empty() && lds[i][r] > gmin + gmin / 4 + 1024) {
+            if (!g.reps.empty() && (lds[i][r] > gmin + gmin / 4 + 1024 || dims[i][r].w != g.variant)) {
                 groups.push_back(g);
                 g = Group();
                 g.stage = i;
             }
-            if (g.reps.empty()) gmin = lds[i][r];
+            if (g.reps.empty()) {
+                gmin = lds[i][r];
+                g.variant = dims[i][r].w;
+            }
             g.reps.push_back(r);
             if (lds[i][r] > g.lds) g.lds = lds[i][r];
         }
@@ -251,6 +259,7 @@ static int seg_prepare(mdf_batch *b) {
         u.S.first = (const int *)(base + align(st.arg_size * n));
         u.S.dims = (const int4 *)(base + align(st.arg_size * n) + align(sizeof(int) * (n + 1)));
         u.S.nrep = (int)n;
+        u.S.variant = g.variant;
         u.total = g.total;
         u.lds = g.lds;
         b->runs.push_back(u);
@@ -444,16 +453,22 @@ int mdf_batch_run(mdf_batch *b, double *const *pos, double *const *vel, double *
             return MDF_ESTATE;
         }
     }
+    // replicas with pair lists build at their first step: the caller may have
+    // moved the particles since the last call (inside the graph the list
+    // decisions are the device's own)
+    for (int r = 0; r < b->nrep; ++r) MDF_TRY(vlist_request_build(b->ctx[r], n[r], s));
     const int K = graph_steps > 0 ? (graph_steps < nsteps ? graph_steps : nsteps) : 0;
     const int nfull = K > 0 ? nsteps / K : 0;
     if (nfull > 0) {
         // (re)capture when the arguments differ from the captured ones
         bool same = b->exec && b->K == K && b->nq == nq && b->dt == dt && b->af == af && b->uf == uf &&
                     b->kf == kf;
+        // (a replica rebuilt in place since, by mdf_p3m_set_skin or
+        // mdf_p3m_set_launch, has new buffers: capture again)
         for (int r = 0; same && r < b->nrep; ++r)
             same = b->cap_pos[r] == pos[r] && b->cap_vel[r] == vel[r] && b->cap_acc[r] == acc[r] &&
                    b->cap_n[r] == n[r] && b->cap_e[r] == (energies ? energies[r] : nullptr) &&
-                   b->cap_q[r] == (nq > 0 ? nqt[r] : nullptr);
+                   b->cap_q[r] == (nq > 0 ? nqt[r] : nullptr) && b->cap_gen[r] == b->ctx[r]->gen;
         if (!same) {
             const size_t ne = (size_t)b->nrep * K * 4, nqn = (size_t)b->nrep * K * (nq > 0 ? nq : 1) * 6;
             if (ne > b->ring_e_cap) {
@@ -480,6 +495,8 @@ int mdf_batch_run(mdf_batch *b, double *const *pos, double *const *vel, double *
             b->cap_vel.assign(vel, vel + b->nrep);
             b->cap_acc.assign(acc, acc + b->nrep);
             b->cap_n.assign(n, n + b->nrep);
+            b->cap_gen.resize(b->nrep);
+            for (int r = 0; r < b->nrep; ++r) b->cap_gen[r] = b->ctx[r]->gen;
             b->cap_e.assign(b->nrep, nullptr);
             b->cap_q.assign(b->nrep, nullptr);
             std::vector<mdf_batch::Flush> tab(b->nrep);

@@ -269,14 +269,21 @@ int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hip
 }
 
 // ---- segmented stages (replica batches) ----
+// With pair lists the binning returns at once on a list step (the mode the
+// decision stage wrote), as in bin_step.
+static BinArgs seg_bin_args(mdf_p3m_ctx *c, const StepIO &io) {
+    BinArgs A = bin_args(c, io.pos, nullptr, io.n);
+    if (c->vl_skin > 0.0 && io.n > 0) A.mode = c->vl_flags + 3;
+    return A;
+}
 static int build_bin_particles(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
-    *(BinArgs *)args = bin_args(c, io.pos, nullptr, io.n);
+    *(BinArgs *)args = seg_bin_args(c, io);
     dims = make_int4((io.n + 255) / 256 > 0 ? (io.n + 255) / 256 : 1, 1, 1, 0);
     lds = 0;
     return MDF_OK;
 }
 static int build_bin_tiles(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
-    const BinArgs A = bin_args(c, io.pos, nullptr, io.n);
+    const BinArgs A = seg_bin_args(c, io);
     *(BinArgs *)args = A;
     dims = make_int4(A.ntile, 1, 1, 0);
     lds = 0;

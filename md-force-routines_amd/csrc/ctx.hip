@@ -150,7 +150,9 @@ namespace mdf {
 // cutoff stay rc.  skin > 0 also allocates the Verlet pair lists.
 // req: PP waves per workgroup, assignment and gather block shapes (1-based
 // indices into the shape list below); 0 leaves the choice to this function.
-static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], mdf_p3m_ctx **out) {
+static double list_skin_of(const mdf_p3m_ctx *c, double skin, double cap);
+
+static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], double list_cap, mdf_p3m_ctx **out) {
     if (!p || !out) {
         set_error("mdf_p3m_create: null argument");
         return MDF_EINVAL;
@@ -486,18 +488,11 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], md
         free_ctx(c);
         return r;
     }
-    // the lists take the largest skin the cells allow: every pair within the
-    // smallest cell width (L / cells, at least rc + skin) is inside the stencil
-    double skin_eff = 0.0;
-    if (skin > 0.0) {
-        double wmin = 2.0 * g.w[0];
-        for (int d = 1; d < 3; ++d) wmin = std::min(wmin, 2.0 * g.w[d]);
-        skin_eff = wmin - p->rc;
-    }
-    if ((r = vlist_alloc(c, skin_eff)) != MDF_OK) {
+    if ((r = vlist_alloc(c, list_skin_of(c, skin, list_cap))) != MDF_OK) {
         free_ctx(c);
         return r;
     }
+    c->list_skin_req = list_cap;
     c->req_waves = req[0];
     c->req_assign = req[1];
     c->req_gather = req[2];
@@ -517,19 +512,30 @@ extern "C" {
 
 int mdf_p3m_create(const mdf_p3m_params *p, mdf_p3m_ctx **out) {
     static const int automatic[3] = {0, 0, 0};
-    return create_ctx(p, 0.0, automatic, out);
+    return create_ctx(p, 0.0, automatic, 0.0, out);
 }
 
 }  // extern "C"
 
 namespace mdf {
+// The lists' skin of a context whose cells were sized for rc + skin: the
+// largest the cells allow (every pair within the smallest cell width, at
+// least rc + skin, is inside the stencil), or at most cap when cap > 0.
+static double list_skin_of(const mdf_p3m_ctx *c, double skin, double cap) {
+    if (!(skin > 0.0)) return 0.0;
+    double wmin = 2.0 * c->g.w[0];
+    for (int d = 1; d < 3; ++d) wmin = std::min(wmin, 2.0 * c->g.w[d]);
+    const double s = wmin - c->g.rc;
+    return cap > 0.0 && cap < s ? cap : s;
+}
+
 // Rebuild a context with another skin or launch choice: a new context is
 // created, the Green function and the attached network carry over, then the
 // two swap contents so that the caller's handle stays valid.  On failure the
 // context is unchanged.
 static int rebuild_ctx(mdf_p3m_ctx *ctx, double skin, const int req[3], const char *who) {
     mdf_p3m_ctx *nc = nullptr;
-    MDF_TRY(create_ctx(&ctx->prm, skin, req, &nc));
+    MDF_TRY(create_ctx(&ctx->prm, skin, req, ctx->list_skin_req, &nc));
     if (ctx->have_green) {
         hipError_t e = hipMemcpy(nc->green, ctx->green, ctx->nspec * sizeof(double), hipMemcpyDeviceToDevice);
         if (e == hipSuccess && ctx->greenf)
@@ -542,6 +548,7 @@ static int rebuild_ctx(mdf_p3m_ctx *ctx, double skin, const int req[3], const ch
         nc->have_green = true;
     }
     nc->mlp = ctx->mlp;
+    nc->gen = ctx->gen + 1;
     (void)hipDeviceSynchronize();
     mdf_p3m_ctx tmp;
     std::memcpy((void *)&tmp, (void *)ctx, sizeof tmp);
@@ -562,6 +569,22 @@ int mdf_p3m_set_skin(mdf_p3m_ctx *ctx, double skin) {
     if (skin == ctx->skin_req) return MDF_OK;
     const int req[3] = {ctx->req_waves, ctx->req_assign, ctx->req_gather};
     return rebuild_ctx(ctx, skin, req, "mdf_p3m_set_skin");
+}
+
+int mdf_p3m_set_list_skin(mdf_p3m_ctx *ctx, double list_skin, double *effective) {
+    if (!ctx || !(list_skin >= 0.0) || !(list_skin < 1e300)) {
+        set_error("mdf_p3m_set_list_skin: need a context and a finite list skin >= 0");
+        return MDF_EINVAL;
+    }
+    ctx->list_skin_req = list_skin;
+    const double s = list_skin_of(ctx, ctx->skin_req, list_skin);
+    if (s != ctx->vl_skin) {
+        MDF_HIP(hipDeviceSynchronize());  // the lists of earlier calls are released
+        MDF_TRY(vlist_alloc(ctx, s));
+        ctx->gen += 1;
+    }
+    if (effective) *effective = ctx->vl_skin;
+    return MDF_OK;
 }
 
 int mdf_p3m_set_launch(mdf_p3m_ctx *ctx, int pp_waves, int assign_shape, int gather_shape) {

@@ -593,7 +593,9 @@ void seg_stages_md(std::vector<SegStage> &out, int part) {
 int seg_step_stages(int precision, std::vector<SegStage> &out) {
     out.clear();
     seg_stages_md(out, 0);
+    seg_stages_vl(out, 0);
     seg_stages_bin(out);
+    seg_stages_vl(out, 1);
     seg_stages_pp(out, precision);
     seg_stages_pm(out, precision, 0);
     seg_stages_fft(out, precision);

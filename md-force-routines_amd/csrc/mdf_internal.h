@@ -164,6 +164,7 @@ struct mdf_p3m_ctx {
     // while no particle has moved more than skin / 2.  On a list step spos
     // holds the step's positions in the frozen order.
     double vl_skin;
+    double list_skin_req;    // cap on vl_skin (mdf_p3m_set_list_skin; 0: the largest the cells allow)
     int vl_cap;              // list entries per particle
     unsigned short *vl_list; // n_max * vl_cap
     int *vl_cnt;             // n_max
@@ -172,6 +173,7 @@ struct mdf_p3m_ctx {
                              // (1 build, 2 lists), [4] builds, [5] list steps (counts)
     int vl_force_build;      // host: the next list-mode step must build
     int vl_n;                // host: particle count of the last step prepared (a change builds)
+    int gen;                 // host: in-place rebuilds so far (captured graphs hold its buffers)
 };
 
 namespace mdf {
@@ -268,6 +270,9 @@ int unpermute3(mdf_p3m_ctx *c, const double *src_sorted, double *dst, int n, hip
 // records the build positions.  pos: caller order.
 int bin_step(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s);
 int vlist_alloc(mdf_p3m_ctx *c, double skin);
+// Requests a build at the next list-aware step from the stream (outside a
+// captured graph: the particles of n may have moved since the last step).
+int vlist_request_build(mdf_p3m_ctx *c, int n, hipStream_t s);
 void vlist_free(mdf_p3m_ctx *c);
 
 // Dynamic LDS above 64 KiB (gfx950 has 160 KiB per workgroup) must be
@@ -297,8 +302,9 @@ __device__ __forceinline__ Blk blk_plain() {
 // its own grid dims[r] (flattened x fastest).
 struct SegGrid {
     const int *first;
-    const int4 *dims;  // (nx, ny, nz, -)
+    const int4 *dims;  // (nx, ny, nz, variant)
     int nrep;
+    int variant;       // host: the kernel variant of the launch (dims[r].w of its replicas)
 };
 __device__ __forceinline__ int seg_locate(const SegGrid &S, Blk &b) {
     const int g = (int)blockIdx.x;
@@ -342,6 +348,7 @@ void seg_stages_bin(std::vector<SegStage> &out);
 void seg_stages_pp(std::vector<SegStage> &out, int precision);
 void seg_stages_pm(std::vector<SegStage> &out, int precision, int part);  // 0: fill + assign, 1: gather
 void seg_stages_fft(std::vector<SegStage> &out, int precision);
+void seg_stages_vl(std::vector<SegStage> &out, int part);  // 0: prepare + decide, 1: refresh
 
 inline int pp_table_pieces(const mdf_p3m_ctx *c) {
     return c->prm.precision == MDF_PREC_MIXED ? c->sp.npiece : c->npiece;

@@ -787,12 +787,14 @@ void k_pp(PPArgs<R> P) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
     pp_body<R, NW, LISTS>(P, blk_plain(), ldsraw);
 }
-template <class R, int NW>
+// LISTS: a launch whose replicas all have pair lists (the segmented PP stage
+// launches the replicas with and without lists apart).
+template <class R, int NW, bool LISTS>
 __global__ __launch_bounds__(NW * 64) void k_pp_seg(const PPArgs<R> *__restrict__ tab, SegGrid S) {
     extern __shared__ __align__(16) unsigned char ldsraw[];
     Blk b;
     const int r = seg_locate(S, b);
-    pp_body<R, NW, false>(tab[r], b, ldsraw);
+    pp_body<R, NW, LISTS>(tab[r], b, ldsraw);
 }
 
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw) {
@@ -875,10 +877,10 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
         set_error("replica batch: mixed precisions");
         return MDF_EINVAL;
     }
-    // replica batches step without pair lists
-    *(PPArgs<R> *)args = pp_args<R>(c, sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->stab
-                                                    : (const typename Vec2<R>::t *)c->cheb, false);
-    dims = make_int4(c->nblock, 1, 1, 0);
+    const PPArgs<R> P = pp_args<R>(c, sizeof(R) == 4 ? (const typename Vec2<R>::t *)c->stab
+                                                     : (const typename Vec2<R>::t *)c->cheb, true);
+    *(PPArgs<R> *)args = P;
+    dims = make_int4(c->nblock, 1, 1, P.vl_flags ? 1 : 0);  // variant 1: the list kernel
     // segmented launches run 16 waves for tiles above 64 KB, else 8 (FP32) or
     // 4 (FP64), whatever the replica's own launch uses; the LDS is sized for that
     const bool mixed = sizeof(R) == 4;
@@ -894,20 +896,25 @@ static int build_pp(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size
 // replica, whose reservation also covers the 16-wave queues of any 4- or
 // 8-wave replica beside it (at most 64 + 6 KB).
 static_assert(kGroups * kQCap * 4 == 512, "per-wave PP scratch is 512 B");
-template <class R>
-static int launch_pp_seg(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
+template <class R, bool LISTS>
+static int launch_pp_seg_k(const void *tab, const SegGrid &S, int total, size_t lds, hipStream_t s) {
     if (lds > 68 * 1024) {
-        MDF_TRY(allow_lds(k_pp_seg<R, 16>, lds));
-        hipLaunchKernelGGL((k_pp_seg<R, 16>), dim3(total), dim3(16 * 64), lds, s, (const PPArgs<R> *)tab, S);
+        MDF_TRY(allow_lds(k_pp_seg<R, 16, LISTS>, lds));
+        hipLaunchKernelGGL((k_pp_seg<R, 16, LISTS>), dim3(total), dim3(16 * 64), lds, s, (const PPArgs<R> *)tab, S);
     } else if (sizeof(R) == 4) {
-        MDF_TRY(allow_lds(k_pp_seg<R, 8>, lds));
-        hipLaunchKernelGGL((k_pp_seg<R, 8>), dim3(total), dim3(8 * 64), lds, s, (const PPArgs<R> *)tab, S);
+        MDF_TRY(allow_lds(k_pp_seg<R, 8, LISTS>, lds));
+        hipLaunchKernelGGL((k_pp_seg<R, 8, LISTS>), dim3(total), dim3(8 * 64), lds, s, (const PPArgs<R> *)tab, S);
     } else {
-        MDF_TRY(allow_lds(k_pp_seg<R, 4>, lds));
-        hipLaunchKernelGGL((k_pp_seg<R, 4>), dim3(total), dim3(4 * 64), lds, s, (const PPArgs<R> *)tab, S);
+        MDF_TRY(allow_lds(k_pp_seg<R, 4, LISTS>, lds));
+        hipLaunchKernelGGL((k_pp_seg<R, 4, LISTS>), dim3(total), dim3(4 * 64), lds, s, (const PPArgs<R> *)tab, S);
     }
     MDF_LAUNCHED();
     return MDF_OK;
+}
+template <class R>
+static int launch_pp_seg(const void *tab, const SegGrid &S, int total, size_t lds, int, hipStream_t s) {
+    if (S.variant == 1) return launch_pp_seg_k<R, true>(tab, S, total, lds, s);
+    return launch_pp_seg_k<R, false>(tab, S, total, lds, s);
 }
 
 void seg_stages_pp(std::vector<SegStage> &out, int precision) {

@@ -34,26 +34,33 @@ namespace mdf {
 constexpr int kPrepThreads = 256;
 constexpr int kPrepPerThread = 4;  // particles a thread: the block maxima stay few
 
-__global__ __launch_bounds__(kPrepThreads) void k_vl_prep(const double *__restrict__ pos, int n,
-                                                          const int *__restrict__ perm,
-                                                          const double4 *__restrict__ vl_ref,
-                                                          double4 *__restrict__ spos, double Lx, double Ly, double Lz,
-                                                          const int *__restrict__ flags, float *__restrict__ bmax) {
-    __shared__ float wmax[kPrepThreads / 64];
-    if (flags[0]) return;  // a build is requested: the frozen order is replaced anyway
+struct VlPrepArgs {
+    const double *pos;
+    int n;
+    const int *perm;
+    const double4 *vl_ref;
+    double4 *spos;
+    double L[3];
+    const int *flags;
+    float *bmax;
+};
+
+__device__ __forceinline__ void vl_prep_body(const VlPrepArgs &A, int bx, float *wmax) {
+    if (A.flags[0]) return;  // a build is requested: the frozen order is replaced anyway
+    const double Lx = A.L[0], Ly = A.L[1], Lz = A.L[2];
     float d2f = 0.0f;
     for (int q = 0; q < kPrepPerThread; ++q) {
-        const int s = (blockIdx.x * kPrepPerThread + q) * kPrepThreads + threadIdx.x;
-        if (s >= n) break;
-        const int i = perm[s];
-        const double4 r = vl_ref[s];
-        double dx = pos[3 * (size_t)i + 0] - r.x;
-        double dy = pos[3 * (size_t)i + 1] - r.y;
-        double dz = pos[3 * (size_t)i + 2] - r.z;
+        const int s = (bx * kPrepPerThread + q) * kPrepThreads + threadIdx.x;
+        if (s >= A.n) break;
+        const int i = A.perm[s];
+        const double4 r = A.vl_ref[s];
+        double dx = A.pos[3 * (size_t)i + 0] - r.x;
+        double dy = A.pos[3 * (size_t)i + 1] - r.y;
+        double dz = A.pos[3 * (size_t)i + 2] - r.z;
         dx -= Lx * rint(dx / Lx);
         dy -= Ly * rint(dy / Ly);
         dz -= Lz * rint(dz / Lz);
-        spos[s] = make_double4(r.x + dx, r.y + dy, r.z + dz, r.w);
+        A.spos[s] = make_double4(r.x + dx, r.y + dy, r.z + dz, r.w);
         const double d2 = dx * dx + dy * dy + dz * dz;
         d2f = fmaxf(d2f, d2 <= 1e30 ? (float)d2 : 1e30f);  // a non-finite position forces a build
     }
@@ -65,28 +72,44 @@ __global__ __launch_bounds__(kPrepThreads) void k_vl_prep(const double *__restri
     if (threadIdx.x == 0) {
         float m = wmax[0];
         for (int w = 1; w < kPrepThreads / 64; ++w) m = fmaxf(m, wmax[w]);
-        bmax[blockIdx.x] = m;
+        A.bmax[bx] = m;
     }
 }
 
+__global__ __launch_bounds__(kPrepThreads) void k_vl_prep(VlPrepArgs A) {
+    __shared__ float wmax[kPrepThreads / 64];
+    vl_prep_body(A, blockIdx.x, wmax);
+}
+__global__ __launch_bounds__(kPrepThreads) void k_vl_prep_seg(const VlPrepArgs *__restrict__ tab, SegGrid S) {
+    __shared__ float wmax[kPrepThreads / 64];
+    Blk b;
+    const int r = seg_locate(S, b);
+    vl_prep_body(tab[r], b.x, wmax);
+}
+
 // The decision over the block maxima of k_vl_prep (nb of them; skipped when a
-// build is already requested): one block, its thread 0 writes.
-__global__ __launch_bounds__(1024) void k_vl_decide(int *__restrict__ flags, const float *__restrict__ bmax, int nb,
-                                                    float thr) {
-    __shared__ float wmax[16];
-    __shared__ int forced;
+// build is already requested): one block of 1024 threads, its thread 0 writes.
+struct VlDecideArgs {
+    int *flags;
+    const float *bmax;
+    int nb;
+    float thr;
+};
+
+__device__ __forceinline__ void vl_decide_body(const VlDecideArgs &A, float *wmax, int &forced) {
+    int *flags = A.flags;
     if (threadIdx.x == 0) forced = flags[0];
     __syncthreads();
     float m = 0.0f;
     if (!forced)
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fmaxf(m, bmax[b]);
+        for (int b = threadIdx.x; b < A.nb; b += blockDim.x) m = fmaxf(m, A.bmax[b]);
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x != 0) return;
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wmax[w]);
     m = fmaxf(m, wmax[0]);
-    const bool build = forced != 0 || flags[1] != 0 || m > thr;
+    const bool build = forced != 0 || flags[1] != 0 || m > A.thr;
     flags[3] = build ? 1 : 2;
     flags[0] = 0;
     if (build) {
@@ -97,15 +120,56 @@ __global__ __launch_bounds__(1024) void k_vl_decide(int *__restrict__ flags, con
     }
 }
 
-__global__ __launch_bounds__(256) void k_vl_refresh(const double4 *__restrict__ spos, int n,
-                                                    double4 *__restrict__ vl_ref, const int *__restrict__ flags) {
-    if (flags[3] != 1) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) vl_ref[t] = spos[t];
+__global__ __launch_bounds__(1024) void k_vl_decide(VlDecideArgs A) {
+    __shared__ float wmax[16];
+    __shared__ int forced;
+    vl_decide_body(A, wmax, forced);
+}
+__global__ __launch_bounds__(1024) void k_vl_decide_seg(const VlDecideArgs *__restrict__ tab, SegGrid S) {
+    __shared__ float wmax[16];
+    __shared__ int forced;
+    Blk b;
+    const int r = seg_locate(S, b);
+    vl_decide_body(tab[r], wmax, forced);
+}
+
+struct VlRefreshArgs {
+    const double4 *spos;
+    int n;
+    double4 *vl_ref;
+    const int *flags;
+};
+
+__device__ __forceinline__ void vl_refresh_body(const VlRefreshArgs &A, int bx) {
+    if (A.flags[3] != 1) return;
+    const int t = bx * blockDim.x + threadIdx.x;
+    if (t < A.n) A.vl_ref[t] = A.spos[t];
+}
+__global__ __launch_bounds__(256) void k_vl_refresh(VlRefreshArgs A) { vl_refresh_body(A, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_vl_refresh_seg(const VlRefreshArgs *__restrict__ tab, SegGrid S) {
+    Blk b;
+    const int r = seg_locate(S, b);
+    vl_refresh_body(tab[r], b.x);
 }
 
 __global__ void k_vl_set(int *__restrict__ p, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+static int prep_blocks(int n) { return (n + kPrepThreads * kPrepPerThread - 1) / (kPrepThreads * kPrepPerThread); }
+
+// Build when some particle moved more than skin / 2 (the float threshold
+// rounded down, so that an equal distance builds).
+static float vl_threshold(const mdf_p3m_ctx *c) {
+    const double h = 0.5 * c->vl_skin;
+    float thr = (float)(h * h);
+    if ((double)thr > h * h) thr = nextafterf(thr, 0.0f);
+    return thr;
+}
+
+static VlPrepArgs prep_args(mdf_p3m_ctx *c, const double *pos, int n) {
+    return VlPrepArgs{pos,       n, c->perm, c->vl_ref, c->spos, {c->g.L[0], c->g.L[1], c->g.L[2]}, c->vl_flags,
+                      (float *)c->part_red};  // part_red is free until the n(q) stage of the step
 }
 
 int bin_step(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s) {
@@ -119,27 +183,81 @@ int bin_step(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStrea
         MDF_LAUNCHED();
         c->vl_force_build = 0;
     }
-    const int per_block = kPrepThreads * kPrepPerThread;
-    const int nb = (n + per_block - 1) / per_block;
+    const int nb = prep_blocks(n);
     if (nb > kMaxPartials) {
         set_error("pair lists: %d particles exceed the displacement partials", n);
         return MDF_EINVAL;
     }
-    float *bmax = (float *)c->part_red;  // free until the n(q) stage of the step
-    hipLaunchKernelGGL(k_vl_prep, dim3(nb), dim3(kPrepThreads), 0, s, pos, n, c->perm, c->vl_ref, c->spos,
-                       c->g.L[0], c->g.L[1], c->g.L[2], c->vl_flags, bmax);
+    const VlPrepArgs A = prep_args(c, pos, n);
+    hipLaunchKernelGGL(k_vl_prep, dim3(nb), dim3(kPrepThreads), 0, s, A);
     MDF_LAUNCHED();
-    // build when some particle moved more than skin / 2 (the float threshold
-    // rounded down, so that an equal distance builds)
-    const double h = 0.5 * c->vl_skin;
-    float thr = (float)(h * h);
-    if ((double)thr > h * h) thr = nextafterf(thr, 0.0f);
-    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(1024), 0, s, c->vl_flags, bmax, nb, thr);
+    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(1024), 0, s, VlDecideArgs{c->vl_flags, A.bmax, nb, vl_threshold(c)});
     MDF_LAUNCHED();
     MDF_TRY(bin_particles_mode(c, pos, Z, n, c->vl_flags + 3, s));
-    hipLaunchKernelGGL(k_vl_refresh, dim3((n + 255) / 256), dim3(256), 0, s, c->spos, n, c->vl_ref, c->vl_flags);
+    hipLaunchKernelGGL(k_vl_refresh, dim3((n + 255) / 256), dim3(256), 0, s, VlRefreshArgs{c->spos, n, c->vl_ref, c->vl_flags});
     MDF_LAUNCHED();
     return MDF_OK;
+}
+
+int vlist_request_build(mdf_p3m_ctx *c, int n, hipStream_t s) {
+    if (!(c->vl_skin > 0.0)) return MDF_OK;
+    hipLaunchKernelGGL(k_vl_set, dim3(1), dim3(64), 0, s, c->vl_flags, 1);
+    MDF_LAUNCHED();
+    c->vl_force_build = 0;
+    c->vl_n = n;
+    return MDF_OK;
+}
+
+// ---- segmented stages (replica batches) ----
+// A replica without pair lists has no blocks in these stages.
+static bool seg_lists(const mdf_p3m_ctx *c, const StepIO &io) { return c->vl_skin > 0.0 && io.n > 0; }
+
+static int build_vl_prep(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    const bool on = seg_lists(c, io);
+    const int nb = on ? prep_blocks(io.n) : 0;
+    if (nb > kMaxPartials) {
+        set_error("pair lists: %d particles exceed the displacement partials", io.n);
+        return MDF_EINVAL;
+    }
+    *(VlPrepArgs *)args = prep_args(c, io.pos, io.n);
+    dims = make_int4(nb, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int build_vl_decide(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    const bool on = seg_lists(c, io);
+    *(VlDecideArgs *)args = VlDecideArgs{c->vl_flags, (const float *)c->part_red, on ? prep_blocks(io.n) : 0,
+                                         on ? vl_threshold(c) : 0.0f};
+    dims = make_int4(on ? 1 : 0, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+static int build_vl_refresh(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims, size_t &lds) {
+    const bool on = seg_lists(c, io);
+    *(VlRefreshArgs *)args = VlRefreshArgs{c->spos, io.n, c->vl_ref, c->vl_flags};
+    dims = make_int4(on ? (io.n + 255) / 256 : 0, 1, 1, 0);
+    lds = 0;
+    return MDF_OK;
+}
+#define MDF_SEG_LAUNCH(fname, kern, threads, Args)                                                        \
+    static int fname(const void *tab, const SegGrid &S, int total, size_t, int, hipStream_t s) {        \
+        if (total == 0) return MDF_OK; /* no replica with pair lists */                                   \
+        hipLaunchKernelGGL(kern, dim3(total), dim3(threads), 0, s, (const Args *)tab, S);                   \
+        MDF_LAUNCHED();                                                                                     \
+        return MDF_OK;                                                                                      \
+    }
+MDF_SEG_LAUNCH(launch_vl_prep, k_vl_prep_seg, kPrepThreads, VlPrepArgs)
+MDF_SEG_LAUNCH(launch_vl_decide, k_vl_decide_seg, 1024, VlDecideArgs)
+MDF_SEG_LAUNCH(launch_vl_refresh, k_vl_refresh_seg, 256, VlRefreshArgs)
+#undef MDF_SEG_LAUNCH
+
+void seg_stages_vl(std::vector<SegStage> &out, int part) {
+    if (part == 0) {
+        out.push_back({"vl_prep", sizeof(VlPrepArgs), build_vl_prep, launch_vl_prep});
+        out.push_back({"vl_decide", sizeof(VlDecideArgs), build_vl_decide, launch_vl_decide});
+    } else {
+        out.push_back({"vl_refresh", sizeof(VlRefreshArgs), build_vl_refresh, launch_vl_refresh});
+    }
 }
 
 void vlist_free(mdf_p3m_ctx *c) {

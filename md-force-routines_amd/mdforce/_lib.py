@@ -69,6 +69,7 @@ SIGNATURES = {
     "mdf_md_run": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _i, _d, _vp, _vp, _vp],
     "mdf_p3m_stats": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_set_skin": [_vp, _d],
+    "mdf_p3m_set_list_skin": [_vp, _d, ct.POINTER(ct.c_double)],
     "mdf_p3m_list_stats": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_config": [_vp, ct.POINTER(ct.c_int)],
     "mdf_p3m_set_launch": [_vp, ct.c_int, ct.c_int, ct.c_int],

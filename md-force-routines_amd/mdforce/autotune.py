@@ -12,10 +12,14 @@ budget is stated in: sqrt(mean_i |F_i - F_i^ref|^2) / mean_i |F_i^ref|.
 Cost model (seconds per step), fitted to the N = 1e6, 128^3, rc 7 stage
 times of the FP32-mixed engine on MI355X (PP 1.60 ms, assignment + gather
 0.37 ms, FFT solve 0.106 ms):
-    t = a_pp * N * rc^3 + a_pm * N + a_fft * M^3 * log2(M^3) / 21 + t_0
+    t = a_pp * N * rc^3 + a_pm * N + a_fft * M^3 * log2(M^3) / 21 + a_mesh * M^3 + t_0
 with a_pp = 4.7e-12 (PP at unit density: rc^3 in-cutoff pairs per
 particle), a_pm = 3.7e-10 (assignment + gather), a_fft = 5.0e-11 (FFT
-solve per mesh node at M = 128, scaled by log2(M^3)/21), t_0 = 2e-5 (launch
+solve per mesh node at M = 128, scaled by log2(M^3)/21), a_mesh = 3e-11
+(the assignment's and gather's mesh bricks: clearing, staging and adding
+per node, which dominate for few ions on a fine mesh; fitted to the 64-replica
+ensemble, where moving ten replicas of N = 25,000-50,000 from 64^3 to 128^3
+added 0.6 ms of assignment and gather for 1.8e7 nodes), t_0 = 2e-5 (launch
 overheads per step of a replica).
 
 The PP cell list needs L/rc >= 3, so rc <= L/3.
@@ -42,13 +46,14 @@ class NoCandidateError(RuntimeError):
 A_PP = 4.7e-12
 A_PM = 3.7e-10
 A_FFT = 5.0e-11
+A_MESH = 3.0e-11
 T_0 = 2e-5
 
 
 def model_cost(N, M, rc):
     """Modelled seconds per NVE step of one system."""
     M3 = M ** 3
-    return A_PP * N * rc ** 3 + A_PM * N + A_FFT * M3 * math.log2(M3) / 21.0 + T_0
+    return A_PP * N * rc ** 3 + A_PM * N + A_FFT * M3 * math.log2(M3) / 21.0 + A_MESH * M3 + T_0
 
 
 def force_error(acc, ref):

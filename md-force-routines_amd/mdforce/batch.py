@@ -74,6 +74,86 @@ class ReplicaBatch:
             self.replicas.append(rep)
         self._keep = None
 
+    def set_skins(self, skins):
+        """Pair-list skin per replica (0: none; see P3M.set_skin).  In the
+        segmented graph the replicas with lists share one list-aware PP
+        launch and those without another."""
+        if len(skins) != len(self.replicas):
+            raise ValueError("one skin per replica")
+        for rep, s in zip(self.replicas, skins):
+            rep.set_skin(float(s))
+
+    def apply_skin(self, r, skin):
+        """Pair lists of skin ``skin`` for replica r (0: none): cells of
+        floor(L / (rc + skin)) per axis and the list skin capped at ``skin``.
+        Returns the list skin in use (0 when the box is too small for
+        L / (rc + skin) >= 3)."""
+        from ._lib import MdforceError
+        rep = self.replicas[r]
+        if not skin > 0.0:
+            rep.set_skin(0.0)
+            return 0.0
+        try:
+            rep.set_skin(float(skin))
+        except MdforceError:
+            rep.set_skin(0.0)
+            return 0.0
+        return rep.set_list_skin(float(skin))
+
+    def tune_skins(self, pos, vel, acc, dt, targets=(0.25, 0.5, 1.0), steps=20, nq=0, log=None):
+        """Pair-list skins by measurement, in two passes:
+        1. each replica stepped alone (md_run from copies of its state) with
+           no lists and with each skin of ``targets`` (apply_skin); the
+           fastest is its choice;
+        2. the whole ensemble (graph of ``steps`` steps, run twice from copies
+           of (pos, vel, acc), the second run timed) with those choices and
+           with no lists at all; the faster is kept.
+        Returns the list skin of each replica."""
+        import time
+        from .autotune import time_steps
+        R = len(self.replicas)
+        chosen = []
+        for r in range(R):
+            best = None
+            for s in (0.0,) + tuple(targets):
+                eff = self.apply_skin(r, s)
+                if s > 0.0 and eff == 0.0:
+                    continue
+                t = time_steps(self.replicas[r], pos[r], vel[r], dt, steps=steps)
+                if log:
+                    log(f"replica {r}: skin {eff:.3f} alone {t:.4f} ms/step")
+                if best is None or t < best[1]:
+                    best = (s, t)
+            chosen.append(best[0])
+        P = [t.clone() for t in pos]
+        V = [t.clone() for t in vel]
+        A = [t.clone() for t in acc]
+
+        def trial():
+            for _ in range(2):
+                for r in range(R):
+                    P[r].copy_(pos[r])
+                    V[r].copy_(vel[r])
+                    A[r].copy_(acc[r])
+                torch.cuda.synchronize(self.device)
+                t0 = time.perf_counter()
+                self.run(P, V, A, steps, dt, nq=nq, graph_steps=steps)
+                torch.cuda.synchronize(self.device)
+                t = time.perf_counter() - t0
+            return t * 1e3 / steps
+
+        for r in range(R):
+            self.apply_skin(r, 0.0)
+        t_none = trial()
+        skins = [self.apply_skin(r, s) for r, s in enumerate(chosen)]
+        t_lists = trial()
+        if log:
+            log(f"ensemble: no lists {t_none:.4f} ms/step, chosen skins {t_lists:.4f} ms/step "
+                f"({sum(1 for x in skins if x > 0)} replicas with lists)")
+        if t_none < t_lists:
+            skins = [self.apply_skin(r, 0.0) for r in range(R)]
+        return skins
+
     def set_mode(self, mode):
         """0: segmented launches (one per stage for all replicas); 1: per-replica streams."""
         check(self._lib.mdf_batch_set_mode(self._b, int(mode)), "mdf_batch_set_mode")

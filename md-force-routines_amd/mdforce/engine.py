@@ -120,6 +120,14 @@ class P3M:
         check(self._lib.mdf_p3m_set_skin(self._ctx, float(skin)), "mdf_p3m_set_skin")
         self.skin = float(skin)
 
+    def set_list_skin(self, list_skin):
+        """Caps the lists' skin at ``list_skin`` (0: the largest the cells
+        allow, the default); the cells stay.  Returns the list skin in use
+        (0 without lists)."""
+        eff = ct.c_double(0.0)
+        check(self._lib.mdf_p3m_set_list_skin(self._ctx, float(list_skin), ct.byref(eff)), "mdf_p3m_set_list_skin")
+        return eff.value
+
     def set_launch(self, pp_waves=0, assign_shape=0, gather_shape=0):
         """Launch choices (0 = the context's own): PP waves per workgroup (4, 8,
         12, 16) and the PM block shapes (1 = 8^3 .. 4 = 2^3 sub-cells, gather

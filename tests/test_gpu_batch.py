@@ -32,8 +32,12 @@ def mods():
     return batch, engine
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=["segmented", "streams"])
-def test_batch_graph_matches_single_replicas(mods, mode):
+@pytest.mark.parametrize("mode,skins", [(0, None), (1, None), (0, (0.3, 0.0, 0.5)), (1, (0.3, 0.0, 0.5))],
+                         ids=["segmented", "streams", "segmented-lists", "streams-lists"])
+def test_batch_graph_matches_single_replicas(mods, mode, skins):
+    """With skins, replicas 0 and 2 step on pair lists (in the segmented
+    graph one list-aware PP launch, replica 1 in the other), as do their
+    single contexts."""
     batch, engine = mods
     specs = [(2100, 32, 2.0, 0.8, 4.0, 100.0), (5000, 32, 0.5, 0.6, 5.0, 10.0), (3300, 64, 3.0, 0.9, 3.9, 175.0)]
     systems, states = [], []
@@ -47,17 +51,22 @@ def test_batch_graph_matches_single_replicas(mods, mode):
     for s, (pos, vel) in zip(systems, states):
         e = engine.P3M(s["n"], s["mesh"], s["L"], s["kappa"], s["alpha"], s["rc"])
         e.build_green()
+        if skins:
+            e.set_skin(skins[len(singles)])
         p = torch.from_numpy(pos).cuda()
         v = torch.from_numpy(vel).cuda()
         a, _ = e.force(p)
         a = a.clone()
         en, nqt = e.md_run(p, v, a, nsteps, dt, q_step=2 * math.pi / s["L"], nq=nq)
         torch.cuda.synchronize()
-        singles.append((p.cpu().numpy(), v.cpu().numpy(), a.cpu().numpy(), en.cpu().numpy(), nqt.cpu().numpy()))
+        singles.append((p.cpu().numpy(), v.cpu().numpy(), a.cpu().numpy(), en.cpu().numpy(), nqt.cpu().numpy(),
+                        e.list_stats()))
         e.close()
     # batch (2 graph launches of 4 steps + 2 eager steps)
     b = batch.ReplicaBatch(systems, n_streams=2)
     b.set_mode(mode)
+    if skins:
+        b.set_skins(skins)
     P, V, A = [], [], []
     for rep, (pos, vel) in zip(b.replicas, states):
         p = torch.from_numpy(pos).cuda()
@@ -70,8 +79,15 @@ def test_batch_graph_matches_single_replicas(mods, mode):
     torch.cuda.synchronize()
     assert b.graph_mode == mode
     for r in range(len(systems)):
-        p, v, a, en, nqt = singles[r]
-        print(f"[batch] mode {mode} replica {r}: max|dx| {np.abs(P[r].cpu().numpy() - p).max():.3e}")
+        p, v, a, en, nqt, st1 = singles[r]
+        st = b.replicas[r].list_stats()
+        print(f"[batch] mode {mode} replica {r}: max|dx| {np.abs(P[r].cpu().numpy() - p).max():.3e} "
+              f"lists {st} alone {st1}")
+        if skins and skins[r] > 0:
+            # the same builds and list steps as alone (the force call builds too)
+            assert st["list_steps"] >= 1 and (st["builds"], st["list_steps"]) == (st1["builds"], st1["list_steps"])
+        else:
+            assert st["list_steps"] == 0 and st["list_cap"] == 0
         L = systems[r]["L"]
         dpos = np.abs(P[r].cpu().numpy() - p)
         assert np.minimum(dpos, L - dpos).max() <= 1e-11 * L
@@ -83,6 +99,17 @@ def test_batch_graph_matches_single_replicas(mods, mode):
     ens2, _ = b.run(P, V, A, 8, dt, nq=nq, graph_steps=K)
     torch.cuda.synchronize()
     assert np.isfinite(ens2[0].cpu().numpy()).all()
+    # a replica rebuilt in place (new skin: new buffers) makes the next run
+    # capture again; its energies continue those of the run before
+    b.replicas[1].set_skin(0.4)
+    b.replicas[1].set_list_skin(0.4)
+    ens3, _ = b.run(P, V, A, 8, dt, nq=nq, graph_steps=K)
+    torch.cuda.synchronize()
+    e2, e3 = ens2[1].cpu().numpy(), ens3[1].cpu().numpy()
+    st = b.replicas[1].list_stats()  # (a hot replica: it may build every step)
+    assert np.isfinite(e3).all() and st["list_cap"] > 0 and st["builds"] + st["list_steps"] >= 8
+    # the total energy carries on: its first change no larger than the run before's largest step-to-step change
+    assert abs(e3[0, 0] - e2[-1, 0]) <= 3.0 * np.abs(np.diff(e2[:, 0])).max() + 1e-12 * abs(e2[-1, 0])
     b.close()
 
 

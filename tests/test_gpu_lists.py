@@ -200,3 +200,40 @@ def test_list_steps_after_pm_only_and_charge_calls_rebuild(P3M, system):
     torch.cuda.synchronize()
     assert st["builds"] > b0 and st["list_steps"] >= 2, st
     assert rel_rms(acc.cpu().numpy(), a_ref.cpu().numpy()) <= 1e-12
+
+
+def test_list_skin_cap(P3M, system):
+    """mdf_p3m_set_list_skin caps the lists' skin below the largest the cells
+    allow: fewer entries a list, more builds, the same forces."""
+    S = system
+    P, N = S["P"], S["N"]
+    e = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision="fp64")
+    e.set_green(dev(P.G_k))
+    e.set_skin(0.5)
+    s_max = e.set_list_skin(0.0)
+    cap_max = e.list_stats()["list_cap"]
+    cells = e.list_stats()["cells_x"]
+    assert s_max >= 0.5 and abs(s_max - (S["L"] / cells - S["rc"])) < 1e-12
+    s = e.set_list_skin(0.2)
+    st0 = e.list_stats()
+    assert s == 0.2 and st0["list_cap"] < cap_max and st0["cells_x"] == cells
+    ref = P3M(N, S["M"], S["L"], S["kappa"], S["alpha"], S["rc"], precision="fp64")
+    ref.set_green(dev(P.G_k))
+    pos, vel = dev(S["pos"]), dev(S["vel"])
+    acc, _ = e.force(pos)
+    acc = acc.clone()
+    e.md_run(pos, vel, acc, 12, 0.02)
+    torch.cuda.synchronize()
+    st = e.list_stats()
+    a_ref, _ = ref.force(pos)
+    torch.cuda.synchronize()
+    print(f"[lists] cap 0.2 (largest {s_max:.3f}): {st}")
+    assert st["list_steps"] >= 1 and st["builds"] >= 2
+    assert rel_rms(acc.cpu().numpy(), a_ref.cpu().numpy()) <= 1e-12
+    assert e.set_list_skin(10.0) == s_max  # a cap above the largest skin: the largest
+    e.set_skin(0.0)
+    assert e.set_list_skin(0.2) == 0.0  # no lists
+    with pytest.raises(Exception):
+        e.set_list_skin(-1.0)
+    e.close()
+    ref.close()
