@@ -265,7 +265,11 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], do
     {
         int rr = mixed ? build_pair_table(p->kappa, p->alpha, p->rc, kChebDegF, 1e-7, true, pair_tab, c->npiece)
                        : build_pair_table(p->kappa, p->alpha, p->rc, kChebDeg, 4e-15, false, pair_tab, c->npiece);
-        if (rr == MDF_OK && mixed) rr = build_pair_table_s(p->kappa, p->alpha, p->rc, kSDegF, 1e-7, s_tab, c->sp);
+        // unit of the mixed PP kernel's tile coordinates: 1 / sigma with
+        // sigma = 2^(E/2) / rc in [1, sqrt 2), so that the cutoff is the
+        // squared distance 2^E
+        const int E = (int)ceil(2.0 * log2(p->rc));
+        if (rr == MDF_OK && mixed) rr = build_pair_table_s(p->kappa, p->alpha, p->rc, kSDegF, 1e-7, E, s_tab, c->sp);
         if (rr != MDF_OK) {
             delete c;
             return rr;

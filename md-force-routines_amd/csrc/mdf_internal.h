@@ -66,15 +66,20 @@ struct Brick {
     int lds_gather;    // 1 if the 3-field brick fits LDS for the gather
 };
 
-// FP32 pair table indexed by the float bits of s = r^2 (pair_table.hip): piece
-// p = (bits(s) >> shift) - base for s >= s_lo; the local variable is f - tc,
-// with f the float of s's low `shift` mantissa bits under the exponent of 1
-// and tc the centre of f's range.
+// FP32 pair table indexed by the float bits of s' = sigma^2 r^2, the squared
+// distance in the mixed PP kernel's tile units 1 / sigma (pair_table.hip): piece
+// p = (bits(s') >> shift) - base for s_lo <= s' <= s_top = 2^E = sigma^2 rc^2,
+// the last piece (at s_top) all zeros; the local variable is f - tc, with f
+// the float of s''s low `shift` mantissa bits under the exponent of 1 and tc
+// the centre of f's range.
 struct SPieces {
-    int npiece;
+    int npiece;      // pieces including the zero piece at s_top
     unsigned base;
     int shift;
-    float s_lo, tc;
+    float s_lo, s_top, tc;
+    int E;           // s_top = 2^E
+    double qscale;   // sigma: tile units per length unit
+    float inv_q;     // 1 / sigma
 };
 
 constexpr int kPMThreads = 512;
@@ -226,7 +231,7 @@ int set_green(mdf_p3m_ctx *c, const double *G_full, hipStream_t s);
 // PP pair-term tables (host) and the PP kernel's dynamic LDS size.
 int build_pair_table(double kappa, double alpha, double rc, int deg, double tol, bool as_float,
                      std::vector<double2> &tab, int &npiece);
-int build_pair_table_s(double kappa, double alpha, double rc, int deg, double tol, std::vector<double2> &tab,
+int build_pair_table_s(double kappa, double alpha, double rc, int deg, double tol, int E, std::vector<double2> &tab,
                        SPieces &sp);
 size_t pp_lds_bytes(int tile_max, int npiece, bool mixed, int nw);
 // pieces of the PP table staged in LDS: the s-indexed table (FP32) or the

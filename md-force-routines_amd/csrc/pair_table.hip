@@ -145,55 +145,68 @@ int build_pair_table(double kappa, double alpha, double rc, int deg, double tol,
 
 namespace mdf {
 
-// FP32 table in s = r^2 for the mixed-precision PP kernel.  A piece is 2^-m
-// of an octave of s, so the float bits of s give the piece directly: its
-// exponent and top m mantissa bits, counted from s_lo = 2^-2 (r = 1/2); the
-// low 23 - m mantissa bits give the position t in the piece.  Pieces of equal
-// relative width follow the r^-1 and r^-3 growth of the pair terms at small r
-// with one degree, and no square root is needed to find a pair's piece.  Per
-// piece the table holds, as degree-deg power series of t in [-1, 1),
-//   phi(s) = g1(r) / r      (pair energy)
-//   F(s)   = g2(r) / r^3    (force on i: F (x_i - x_j))
+// FP32 table in the scaled squared distance of the mixed-precision PP
+// kernel.  That kernel keeps tile coordinates in units of 1 / sigma, with
+// sigma = 2^(E/2) / rc, so the table is a function of s' = sigma^2 r^2 and
+// the cutoff r = rc is s' = 2^E exactly.  A piece is 2^-m of an octave of s',
+// so the float bits of s' give the piece directly: its exponent and top m
+// mantissa bits, counted from s_lo (the power of two at or above the s' of
+// r = 1/2); the low 23 - m mantissa bits give the position t in the piece.
+// Pieces of equal relative width follow the r^-1 and r^-3 growth of the pair
+// terms at small r with one degree, and no square root is needed to find a
+// pair's piece.  Per piece the table holds, as degree-deg power series of t in
+// [-1, 1),
+//   phi(s') = g1(r) / r               (pair energy)
+//   F(s')   = g2(r) / r^3 / sigma     (force on i: F (x_i - x_j) sigma)
 // stored for the variable f - tc = t / 2^(m+1) (coefficient k times
 // 2^((m+1) k), exact in float): the kernel forms f - tc with one subtraction,
 // and Horner in it gives t's results bit for bit (power-of-two scalings).
-// fitted at the Chebyshev nodes in long double; the fit with coefficients
-// rounded to float is checked at 64 points per piece, with the same measure as
-// the r-indexed table: the errors of g1 = r phi and g2 = r^3 F against tol
-// times max|g1| and max|g2| on [0, rc]; m grows (to at most 6) until it
-// passes.  Pairs closer than 1/2 (s < s_lo) use the r-indexed table instead.
-int build_pair_table_s(double kappa, double alpha, double rc, int deg, double tol, std::vector<double2> &tab,
+// One more piece, all zeros, starts at s' = 2^E: the kernel clamps s' into
+// [s_lo, 2^E] (one med3), so every pair at or beyond the cutoff reads zeros
+// and needs no cutoff test of its own.  The pieces are fitted at the
+// Chebyshev nodes in long double; the fit with coefficients rounded to float
+// is checked at 64 points per piece, with the same measure as the r-indexed
+// table: the errors of g1 = r phi and g2 = r^3 F (unscaled) against tol times
+// max|g1| and max|g2| on [0, rc]; m grows (to at most 6) until it passes.
+// Pairs closer than s_lo use the r-indexed table instead.
+int build_pair_table_s(double kappa, double alpha, double rc, int deg, double tol, int E, std::vector<double2> &tab,
                        SPieces &sp) {
     const int nc = deg + 1;
     if (nc > 32) {
         set_error("pair-table degree %d too large", deg);
         return MDF_EINVAL;
     }
+    if (E < -60 || E > 120) {
+        set_error("s-indexed pair table: scale exponent %d out of range", E);
+        return MDF_EINVAL;
+    }
     const long double pi = 3.141592653589793238462643383279502884L;
-    const float s_lo = 0.25f;
+    const long double sig2 = ldexpl(1.0L, E) / ((long double)rc * (long double)rc);  // s' = sig2 r^2
+    const long double sig = sqrtl(sig2);
+    const int e_lo = (int)ceill(log2l(0.25L * sig2));
+    const float s_lo = std::ldexp(1.0f, e_lo);
+    const float s_top = std::ldexp(1.0f, E);
     for (int m = 3; m <= 6; ++m) {
         const int shift = 23 - m;
-        unsigned lo_bits;
+        unsigned lo_bits, top_bits;
         std::memcpy(&lo_bits, &s_lo, 4);
-        const unsigned base = lo_bits >> shift;
-        const float s_top = (float)(rc * rc) * (1.0f + 1e-6f);
-        unsigned top_bits;
         std::memcpy(&top_bits, &s_top, 4);
-        const int P = (int)((top_bits >> shift) - base) + 1;
-        tab.assign((size_t)P * nc, make_double2(0.0, 0.0));
-        auto piece_s = [&](int p, long double t) {  // s at t in [-1, 1] of piece p
+        const unsigned base = lo_bits >> shift;
+        const int P = (int)((top_bits >> shift) - base);  // pieces below 2^E; piece P is the zero piece
+        tab.assign((size_t)(P + 1) * nc, make_double2(0.0, 0.0));
+        auto piece_s = [&](int p, long double t) {  // s' at t in [-1, 1] of piece p
             unsigned b0 = (base + (unsigned)p) << shift, b1 = (base + (unsigned)p + 1) << shift;
             float f0, f1;
             std::memcpy(&f0, &b0, 4);
             std::memcpy(&f1, &b1, 4);
             return (long double)f0 + (t + 1.0L) * 0.5L * ((long double)f1 - (long double)f0);
         };
-        auto funcs = [&](long double s, long double &phi, long double &F) {
-            const long double r = sqrtl(s);
+        auto funcs = [&](long double sp_, long double &phi, long double &F) {  // of s'
+            const long double s = sp_ / sig2, r = sqrtl(s);
             long double g1, g2;
             pair_g(r, kappa, alpha, g1, g2);
             phi = g1 / r;
-            F = g2 / (r * s);
+            F = g2 / (r * s) / sig;
         };
         long double g1max = 0.0L, g2max = 0.0L;  // on [0, rc], as the r-indexed table's check
         for (int q = 0; q <= 4096; ++q) {
@@ -226,11 +239,11 @@ int build_pair_table_s(double kappa, double alpha, double rc, int deg, double to
             long double e1 = 0.0L, e2 = 0.0L;
             for (int q = 0; q <= 64; ++q) {
                 const long double t = -1.0L + 2.0L * q / 64.0L;
-                const long double sv = piece_s(p, t), r = sqrtl(sv);
+                const long double sv = piece_s(p, t), s = sv / sig2, r = sqrtl(s);
                 long double a, b;
                 funcs(sv, a, b);
                 e1 = fmaxl(e1, fabsl(horner_ld(c, deg, 0, t) - a) * r);
-                e2 = fmaxl(e2, fabsl(horner_ld(c, deg, 1, t) - b) * r * sv);
+                e2 = fmaxl(e2, fabsl(horner_ld(c, deg, 1, t) - b) * sig * r * s);
             }
             ok = e1 <= (long double)tol * g1max && e2 <= (long double)tol * g2max;
         }
@@ -242,11 +255,15 @@ int build_pair_table_s(double kappa, double alpha, double rc, int deg, double to
                     tab[(size_t)p * nc + k].x *= sc;
                     tab[(size_t)p * nc + k].y *= sc;
                 }
-            sp.npiece = P;
+            sp.npiece = P + 1;
             sp.base = base;
             sp.shift = shift;
             sp.s_lo = s_lo;
+            sp.s_top = s_top;
             sp.tc = 1.0f + std::ldexp(1.0f, -(m + 1));
+            sp.E = E;
+            sp.qscale = (double)sig;
+            sp.inv_q = (float)(1.0L / sig);
             return MDF_OK;
         }
     }

@@ -227,15 +227,56 @@ struct AssignArgs {
 template <bool LDS, class T>
 __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, unsigned char *braw, int *run_start,
                                             int *run_off) {
-    double *brick = (double *)braw;  // FP64 in both modes: FP32 LDS atomics are ~6x slower here
+    // FP64: the brick in FP64 (ds_add_f64).  FP32: the brick in 32-bit fixed
+    // point, each deposit rounded to a multiple of 1/S (ds_add_u32: integer LDS
+    // atomics run ~3x the rate of FP64 ones and ~25x that of FP32 ones here);
+    // S = 2^k is the largest power of two for which no node of the block can
+    // overflow (a deposit is at most 0.167 |Z|, so a node holds at most 0.167
+    // sum|Z| over the block's particles), capped at 2^24.  Each lane carries
+    // the rounding remainder of its deposits into the next one (error
+    // diffusion), so its 36 deposits sum to its share of Z to within half a
+    // unit; the carry starts at a hashed value in [-1/2, 1/2), which makes the
+    // lane's total exactly unbiased (plain rounding drops the many corner
+    // deposits below half a unit and biased the charge low by 2.5e-7 of sum Z
+    // at N = 8e6; a zero start carries the trailing small deposits out of the
+    // sum, 1e-9 low).  The flush adds the brick to the FP64 mesh with FP64
+    // atomics.
+    constexpr bool kFix = LDS && sizeof(T) == 4;
+    double *brick = (double *)braw;
+    int *ibrick = (int *)braw;
     const Geom &g = P.g;
     const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
     const int nb = B0 * B1 * B2;
     const PMBlock blk = pm_block(g, P.br.sb_assign, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) brick[t] = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        if (kFix) ibrick[t] = 0;
+        else brick[t] = 0.0;
+    }
     const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);
     const int npart = run_off[nrun];
+    float kq = 0.0f;     // S / 3840^3: FP32 weight numerators to fixed-point units
+    double inv_s = 0.0;  // charge of one unit (1 / S up to kq's rounding)
+    if constexpr (kFix) {
+        __shared__ float zsum_w[kPMThreads / 64];
+        float zs = 0.0f;
+        for (int k = threadIdx.x; k < npart; k += blockDim.x)
+            zs += fabsf((float)P.spos[pm_particle(run_start, run_off, nrun, k)].w);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) zs += __shfl_xor(zs, o, 64);
+        if ((threadIdx.x & 63) == 0) zsum_w[threadIdx.x >> 6] = zs;
+        __syncthreads();
+        float zt = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) zt += zsum_w[w];
+        // largest k <= 24 with 0.17 zt 2^k <= 2^30 (margin for the rounding of each deposit)
+        int e = 24;
+        const float cap = 0.17f * zt * 1.01f + 1e-30f;
+        while (e > 0 && cap * ldexpf(1.0f, e) > 1073741824.0f) --e;
+        kq = (float)(ldexp(1.0, e) * wscale<T>());
+        // a unit is wscale / kq of charge: kq's FP32 rounding (up to 6e-8)
+        // stays out of the charge sum
+        inv_s = wscale<T>() / (double)kq;
+    }
     // eight lanes a particle, six of them one stencil z-plane each: with half
     // as many particles per wave as the four-lane split of the gather, fewer
     // LDS atomics of one instruction land on the same node (measured ~6%
@@ -243,7 +284,8 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     const int sub = threadIdx.x % kScatterLanes;
     for (int k = threadIdx.x / kScatterLanes; k < npart; k += blockDim.x / kScatterLanes) {
         if (sub >= 6) continue;
-        const double4 p = P.spos[pm_particle(run_start, run_off, nrun, k)];
+        const int si = pm_particle(run_start, run_off, nrun, k);
+        const double4 p = P.spos[si];
         T fx, fy, fz, wx[6], wy[6], wz[6];
         const int ix = base_and_offset<T>(p.x, g.ih[0], fx);
         const int iy = base_and_offset<T>(p.y, g.ih[1], fy);
@@ -258,7 +300,26 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
         for (int a = 1; a < 6; ++a)
             if (sub == a) wza = wz[a];
         const T zw = (T)p.w * wza;
-        if (LDS && inside) {
+        if (kFix && inside) {
+            T wxk[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) wxk[c] = wx[c] * kq;
+            unsigned h = (unsigned)si * 2654435761u + (unsigned)sub * 40503u;
+            h = (h ^ (h >> 15)) * 2246822519u;
+            float carry = (float)(h >> 8) * 5.9604644775390625e-8f - 0.5f;  // 2^-24
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                const T zwy = zw * wy[b];
+                int *row = ibrick + ((rz + sub) * B1 + (ry + b)) * B0 + rx;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const float y = fmaf(zwy, wxk[c], carry);
+                    const float f = floorf(y + 0.5f);
+                    carry = y - f;
+                    atomicAdd(row + c, (int)f);
+                }
+            }
+        } else if (LDS && inside) {
 #pragma unroll
             for (int b = 0; b < 6; ++b) {
                 const T zwy = zw * wy[b];
@@ -282,7 +343,7 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-        const double v = brick[t] * wscale<T>();
+        const double v = kFix ? (double)ibrick[t] * inv_s : brick[t] * wscale<T>();
         if (v != 0.0) {
             const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
             const int gx = wrap3(o0 + tx, g.M[0]);
@@ -327,7 +388,8 @@ int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s) {
     const bool mixed = c->prm.precision == MDF_PREC_MIXED;
     const dim3 grid(c->brick.nblock_assign);
     if (c->brick.lds_assign) {
-        const size_t lds = 8 * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
+        // FP32: the fixed-point brick, 4 B a node (FP64: 8)
+        const size_t lds = (mixed ? 4 : 8) * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2];
         if (mixed) {
             MDF_TRY(allow_lds(k_assign<true, float>, lds));
             hipLaunchKernelGGL((k_assign<true, float>), grid, dim3(kPMThreads), lds, s, P);
@@ -586,7 +648,9 @@ template <class T>
 static int build_assign(mdf_p3m_ctx *c, const StepIO &, void *args, int4 &dims, size_t &lds) {
     *(AssignArgs *)args = assign_args(c);
     dims = make_int4(c->brick.nblock_assign, 1, 1, 0);
-    lds = c->brick.lds_assign ? 8 * (size_t)c->brick.B[0] * c->brick.B[1] * c->brick.B[2] : 0;
+    lds = c->brick.lds_assign ? (c->prm.precision == MDF_PREC_MIXED ? 4 : 8) * (size_t)c->brick.B[0] * c->brick.B[1] *
+                                    c->brick.B[2]
+                              : 0;
     return MDF_OK;
 }
 template <class T>

@@ -77,13 +77,17 @@ struct PPArgs {
     int npiece;                        // pieces of the r-indexed table
     int ntab;                          // pieces of the table staged in LDS (cheb)
     T piece_w, inv_piece_w;
-    // FP32: cheb is the s-indexed table (SPieces); the r-indexed table, for
-    // pairs closer than s_lo, is read from global memory
+    // FP32: tile coordinates are in units of 1 / sigma (sigma = 2^(E/2) / rc,
+    // close to 1), so a pair's s' = sigma^2 r^2 and the cutoff is s' = s_top =
+    // 2^E exactly; cheb is the s-indexed table (SPieces) in s'; the r-indexed
+    // table, for pairs closer than s_lo, is read from global memory
     const T2 *__restrict__ rtab;
     unsigned s_base;
     int s_shift;
-    float s_lo, s_tc;
-    T rc2;                             // rc^2, the pair cutoff
+    float s_lo, s_top, s_tc;
+    double qscale;                     // sigma
+    float qscale_f, inv_q;             // sigma and 1 / sigma in FP32
+    T rc2;                             // rc^2, the pair cutoff (FP32: s_top)
     T rc2m;                            // candidate test radius^2 (rc slightly widened)
     // Verlet pair lists (null vl_flags: off).  vl_flags[3] is the step's mode:
     // 1 = build (the sweep tests candidates within the list radius, evaluates
@@ -96,7 +100,7 @@ struct PPArgs {
     unsigned short *__restrict__ vl_list;  // vl_cap entries per sorted slot: tile slots of its j
     int *__restrict__ vl_cnt;               // entries per sorted slot
     int vl_cap;
-    T rcl2;                                 // list radius^2: (rc + skin)^2, slightly widened
+    T rcl2;                                 // list radius^2: (rc + skin)^2, slightly widened (FP32: times sigma^2)
 };
 
 template <class T>
@@ -215,48 +219,47 @@ __device__ __forceinline__ void pp_pair_masked(const PPArgs<T> &P, const typenam
     fz = fma(fr, dz, fz);
 }
 
-// FP32 pair terms from the s-indexed table (pair_table.hip): the piece and
-// its local variable come from the float bits of s = r^2, and the table holds
-// phi(s) and F(s) = f(r) / r directly, so a pair needs no square root.  Pairs
-// with s < s_lo (r < 1/2: the release of a random start, never the liquid)
-// take the r-indexed table from global memory.  Contributions of pairs that
-// are not ok, outside the cutoff or at r = 0 are selected away (no branch
-// except the rare close-pair one).
-// LIST: a list entry (pp_body's list mode), which is never i itself: lists hold
-// a far dummy slot in place of i and as padding, which the cutoff drops, so
-// only s < rc^2 is tested (and s > 0 only on the rare close-pair path).
-template <bool LIST = false>
-__device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, const float2 *__restrict__ stab, bool ok, float dx,
-                                          float dy, float dz, float &fx, float &fy, float &fz, float &u) {
+// FP32 pair terms from the s-indexed table (pair_table.hip).  The
+// displacement (dx, dy, dz) is in units of 1 / sigma, s' = |d|^2, and the
+// table holds phi and F / sigma as functions of s', so F dx is the force in
+// length units.  The piece and its local variable come from the float bits
+// of s' clamped into [s_lo, s_top] by one med3: s' at or beyond the cutoff
+// (s_top = sigma^2 rc^2 = 2^E) reads the all-zero last piece, which is the
+// cutoff test (candidates beyond rc of a build queue or a list, the far
+// dummy slot, pairs that are not ok evaluated at a far displacement).  Pairs
+// with s' < s_lo (r < 1/2 .. 1/sqrt(2): the release of a random start, never
+// the liquid) and coincident ions take the r-indexed table from global memory
+// behind the one branch.  The sweep's queue holds i itself (s' = 0): that
+// branch drops it too.
+// s_top_v: P.s_top held in a VGPR by the caller (a med3 takes one SGPR operand).
+__device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, unsigned tbase, float s_top_v, float dx, float dy,
+                                          float dz, float &fx, float &fy, float &fz, float &u) {
     constexpr int D = kChebDegF;  // the r-indexed table of the close-pair path
     constexpr int DS = kSDegF;    // the s-indexed table in LDS
     const float s2 = dx * dx + dy * dy + dz * dz;
-    const bool in = LIST ? s2 < P.rc2 : ok && s2 < P.rc2 && s2 > 0.0f;
-    const unsigned b = __float_as_uint(s2);
-    // piece index, clamped with one unsigned min: s < s_lo wraps to the top
-    // piece, whose values the close-pair path below replaces
-    const unsigned p = min((b >> P.s_shift) - P.s_base, (unsigned)P.ntab - 1u);
+    const unsigned b = __float_as_uint(__builtin_amdgcn_fmed3f(s2, P.s_lo, s_top_v));
     const float t = __uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u) - P.s_tc;
+    // piece address: tbase is the table's LDS address less base pieces
     using LdsF2 = __attribute__((address_space(3))) const float2;
-    LdsF2 *c = (LdsF2 *)stab + __umul24(p, kSStride);
+    LdsF2 *c = (LdsF2 *)(size_t)(__umul24(b >> P.s_shift, kSStride * (unsigned)sizeof(float2)) + tbase);
     float phi, F;
     horner2<float, DS>(c, t, phi, F);
-    if (LIST ? s2 < P.s_lo : in && s2 < P.s_lo) {
-        const float ir = rsqrt_t<float>(s2);
-        const float r = s2 * ir;
+    if (s2 < P.s_lo) {
+        const float ir = rsqrt_t<float>(s2);  // 1 / (sigma r)
+        const float r = s2 * ir * P.inv_q;
         const int q = min((int)(r * P.inv_piece_w), P.npiece - 1);
         const float tq = (r - q * P.piece_w) * (2.0f * P.inv_piece_w) - 1.0f;
         float g1, g2;
         horner2<float>(P.rtab + q * (D + 1), tq, g1, g2);
-        phi = g1 * ir;
-        F = g2 * ir * ir * ir;
-        if (LIST && !(s2 > 0.0f)) phi = F = 0.0f;  // coincident ions: no pair term, as the sweep
+        const float irr = ir * P.qscale_f;  // 1 / r
+        phi = g1 * irr;
+        F = g2 * irr * irr * irr * P.inv_q;
+        if (!(s2 > 0.0f)) phi = F = 0.0f;  // i itself, coincident ions: no pair term
     }
-    u += in ? phi : 0.0f;
-    const float Fm = in ? F : 0.0f;
-    fx = fmaf(Fm, dx, fx);
-    fy = fmaf(Fm, dy, fy);
-    fz = fmaf(Fm, dz, fz);
+    u += phi;
+    fx = fmaf(F, dx, fx);
+    fy = fmaf(F, dy, fy);
+    fz = fmaf(F, dz, fz);
 }
 
 __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, double &shift) {
@@ -361,6 +364,13 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
     const double ox = kRel ? (2 * bx + 1) * g.w[0] : 0.0;
     const double oy = kRel ? (2 * by + 1) * g.w[1] : 0.0;
     const double oz = kRel ? (2 * bz + 1) * g.w[2] : 0.0;
+    // FP32: coordinates in units of 1 / sigma
+    const double qs = kRel ? P.qscale : 1.0;
+    // LDS address of the s-indexed table less its first base pieces (FP32)
+    const unsigned tbase = (unsigned)(size_t)(__attribute__((address_space(3))) const R2 *)tab -
+                           (kRel ? P.s_base * (kSStride * (unsigned)sizeof(float2)) : 0u);
+    float s_top_v = kRel ? (float)P.s_top : 0.0f;
+    asm volatile("" : "+v"(s_top_v));
     // 1. counts of the 216 tile sub-cells, tile order x fastest
     if (tid < 216) {
         const int txs = tid % 6, tys = (tid / 6) % 6, tzs = tid / 36;
@@ -455,7 +465,8 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 for (int t = lane; t < ge - gs; t += 64) {
                     const double4 p = P.spos[gs + t];
                     if (kRel) {
-                        const double cx = (p.x + sx) - ox, cy = (p.y + sy) - oy, cz = (p.z + sz) - oz;
+                        const double cx = ((p.x + sx) - ox) * qs, cy = ((p.y + sy) - oy) * qs,
+                                     cz = ((p.z + sz) - oz) * qs;
                         const float hx = (float)cx, hy = (float)cy, hz = (float)cz;
                         const float lx = (float)(cx - (double)hx), ly = (float)(cy - (double)hy),
                                     lz = (float)(cz - (double)hz);
@@ -538,7 +549,8 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 }
             } else {
                 const double4 pi = P.spos[gi];
-                const double cxi = (pi.x + 0.0) - ox, cyi = (pi.y + 0.0) - oy, czi = (pi.z + 0.0) - oz;
+                const double cxi = ((pi.x + 0.0) - ox) * qs, cyi = ((pi.y + 0.0) - oy) * qs,
+                             czi = ((pi.z + 0.0) - oz) * qs;
                 xi = (R)cxi;
                 yi = (R)cyi;
                 zi = (R)czi;
@@ -572,7 +584,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             auto pair_at = [&](int jj) {
                 R dx, dy, dz;
                 disp(jj, dx, dy, dz);
-                if constexpr (kRel) pp_pair_f<false>(P, tab, true, dx, dy, dz, fx, fy, fz, u);
+                if constexpr (kRel) pp_pair_f(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u);
                 else pp_pair_cheb<R>(P, tab, dx, dy, dz, fx, fy, fz, u);
             };
             int qn0 = 0, qn1 = 0;  // queue fill of half 0 / half 1 (wave-uniform)
@@ -610,8 +622,8 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                     disp(j0, dx0, dy0, dz0);
                     disp(j1, dx1, dy1, dz1);
                     if constexpr (kRel) {
-                        pp_pair_f<true>(P, tab, true, dx0, dy0, dz0, fx, fy, fz, u);
-                        pp_pair_f<true>(P, tab, true, dx1, dy1, dz1, fx, fy, fz, u);
+                        pp_pair_f(P, tbase, s_top_v, dx0, dy0, dz0, fx, fy, fz, u);
+                        pp_pair_f(P, tbase, s_top_v, dx1, dy1, dz1, fx, fy, fz, u);
                     } else {
                         pp_pair_masked<R>(P, tab, true, dx0, dy0, dz0, fx, fy, fz, u);
                         pp_pair_masked<R>(P, tab, true, dx1, dy1, dz1, fx, fy, fz, u);
@@ -640,7 +652,7 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         ddz = zi - tz[jc];
                     }
                     const R r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    // FP32 has no self test here: i's own slot gives r^2 = 0
+                    // FP32 has no self test here: i's own slot gives s' = 0
                     // exactly from the hi + lo parts, which the pair drops
                     const bool c1 = r2 < rc2m, c2 = kRel || r2 > (R)0;
                     hit = inrow & c1 & c2;
@@ -817,7 +829,9 @@ static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, bool l
     P.vl_cnt = c->vl_cnt;
     P.vl_cap = c->vl_cap;
     const double rl = (c->g.rc + c->vl_skin) * (1.0 + 1e-6);
-    P.rcl2 = (R)(rl * rl);
+    // FP32 compares squared distances in units of 1 / sigma^2
+    const double q2 = sizeof(R) == 4 ? c->sp.qscale * c->sp.qscale : 1.0;
+    P.rcl2 = (R)(rl * rl * q2);
     P.cheb = cheb;
     P.acc = c->acc_pp_s;
     P.part = c->part_pp;
@@ -831,9 +845,14 @@ static PPArgs<R> pp_args(mdf_p3m_ctx *c, const typename Vec2<R>::t *cheb, bool l
     P.s_base = c->sp.base;
     P.s_shift = c->sp.shift;
     P.s_lo = c->sp.s_lo;
+    P.s_top = c->sp.s_top;
     P.s_tc = c->sp.tc;
+    P.qscale = c->sp.qscale;
+    P.qscale_f = (float)c->sp.qscale;
+    P.inv_q = c->sp.inv_q;
     const double rcm = c->g.rc * (1.0 + 1e-12);
-    P.rc2 = (R)(c->g.rc * c->g.rc);
+    // FP32: the cutoff is s' = s_top = sigma^2 rc^2 exactly (the table's zero piece)
+    P.rc2 = sizeof(R) == 4 ? (R)c->sp.s_top : (R)(c->g.rc * c->g.rc);
     P.rc2m = sizeof(R) == 4 ? P.rc2 : (R)(rcm * rcm);
     return P;
 }
