@@ -262,6 +262,50 @@ __device__ __forceinline__ void pp_pair_f(const PPArgs<float> &P, unsigned tbase
     fz = fmaf(F, dz, fz);
 }
 
+// The table part of pp_pair_f alone (no branch); s' returned for the caller's
+// close-pair bookkeeping.
+__device__ __forceinline__ void pp_pair_tab(const PPArgs<float> &P, unsigned tbase, float s_top_v, float dx, float dy,
+                                            float dz, float &fx, float &fy, float &fz, float &u, float &s2) {
+    s2 = dx * dx + dy * dy + dz * dz;
+    const unsigned b = __float_as_uint(__builtin_amdgcn_fmed3f(s2, P.s_lo, s_top_v));
+    const float t = __uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u) - P.s_tc;
+    using LdsF2 = __attribute__((address_space(3))) const float2;
+    LdsF2 *c = (LdsF2 *)(size_t)(__umul24(b >> P.s_shift, kSStride * (unsigned)sizeof(float2)) + tbase);
+    float phi, F;
+    horner2<float, kSDegF>(c, t, phi, F);
+    u += phi;
+    fx = fmaf(F, dx, fx);
+    fy = fmaf(F, dy, fy);
+    fz = fmaf(F, dz, fz);
+}
+// Correction of a close pair (s' < s_lo) that pp_pair_tab evaluated at the
+// clamped s' = s_lo: its r-indexed terms less the table's.
+__device__ __forceinline__ void pp_pair_fix(const PPArgs<float> &P, unsigned tbase, float s_top_v, float dx, float dy,
+                                            float dz, float &fx, float &fy, float &fz, float &u) {
+    constexpr int D = kChebDegF;
+    const float s2 = dx * dx + dy * dy + dz * dz;
+    if (!(s2 < P.s_lo)) return;
+    const unsigned b = __float_as_uint(__builtin_amdgcn_fmed3f(s2, P.s_lo, s_top_v));
+    const float t = __uint_as_float((b & ((1u << P.s_shift) - 1u)) | 0x3f800000u) - P.s_tc;
+    using LdsF2 = __attribute__((address_space(3))) const float2;
+    LdsF2 *c = (LdsF2 *)(size_t)(__umul24(b >> P.s_shift, kSStride * (unsigned)sizeof(float2)) + tbase);
+    float phi_t, F_t;
+    horner2<float, kSDegF>(c, t, phi_t, F_t);
+    const float ir = rsqrt_t<float>(s2);
+    const float r = s2 * ir * P.inv_q;
+    const int q = min((int)(r * P.inv_piece_w), P.npiece - 1);
+    const float tq = (r - q * P.piece_w) * (2.0f * P.inv_piece_w) - 1.0f;
+    float g1, g2;
+    horner2<float>(P.rtab + q * (D + 1), tq, g1, g2);
+    const float irr = ir * P.qscale_f;
+    float phi = g1 * irr, F = g2 * irr * irr * irr * P.inv_q;
+    if (!(s2 > 0.0f)) phi = F = 0.0f;
+    u += phi - phi_t;
+    fx = fmaf(F - F_t, dx, fx);
+    fy = fmaf(F - F_t, dy, fy);
+    fz = fmaf(F - F_t, dz, fz);
+}
+
 __device__ __forceinline__ void wrap_cell(int c, int n, double L, int &cw, double &shift) {
     if (c < 0) {
         cw = c + n;
@@ -612,21 +656,49 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 const unsigned *lw = (const unsigned *)lst;  // vl_cap is a multiple of 64: dword aligned
                 const int wlast = (cntp >> 1) - 1;
                 auto chunk_word = [&](int c) -> unsigned { return lw[min(c * kGL + gl, wlast)]; };
-                unsigned wa = chunk_word(0), wb = chunk_word(1);
-                for (int c = 0; c < nchunk; ++c) {
-                    const unsigned w = wa;
-                    wa = wb;
-                    wb = chunk_word(c + 2);
+                // FP32: the pair terms come from the table alone (s' clamped to
+                // s_lo at the low end); close pairs (s' < s_lo, the release of
+                // a random start) are corrected in a second pass over the list,
+                // so the loop has no branch for them
+                float smin = 3.0e38f;
+                auto chunk_pairs = [&](unsigned w) {
                     const int j0 = (int)(w & 0xffffu), j1 = (int)(w >> 16);
                     R dx0, dy0, dz0, dx1, dy1, dz1;
                     disp(j0, dx0, dy0, dz0);
                     disp(j1, dx1, dy1, dz1);
                     if constexpr (kRel) {
-                        pp_pair_f(P, tbase, s_top_v, dx0, dy0, dz0, fx, fy, fz, u);
-                        pp_pair_f(P, tbase, s_top_v, dx1, dy1, dz1, fx, fy, fz, u);
+                        float s0, s1;
+                        pp_pair_tab(P, tbase, s_top_v, dx0, dy0, dz0, fx, fy, fz, u, s0);
+                        pp_pair_tab(P, tbase, s_top_v, dx1, dy1, dz1, fx, fy, fz, u, s1);
+                        smin = fminf(smin, fminf(s0, s1));
                     } else {
                         pp_pair_masked<R>(P, tab, true, dx0, dy0, dz0, fx, fy, fz, u);
                         pp_pair_masked<R>(P, tab, true, dx1, dy1, dz1, fx, fy, fz, u);
+                    }
+                };
+                // the list words of the first KPF chunks are loaded up front, so
+                // that their HBM latency overlaps the pair terms of the chunks
+                // before them (a two-ahead register ring made the compiler wait
+                // for each load at the next chunk: 0.773 -> 0.756 ms a list
+                // step, N = 1e6 liquid); longer lists read the rest as they go
+                constexpr int KPF = 8;
+                unsigned wv[KPF];
+#pragma unroll
+                for (int c = 0; c < KPF; ++c) wv[c] = chunk_word(c);
+#pragma unroll
+                for (int c = 0; c < KPF; ++c)
+                    if (c < nchunk) chunk_pairs(wv[c]);
+                for (int c = KPF; c < nchunk; ++c) chunk_pairs(chunk_word(c));
+                if constexpr (kRel) {
+                    if (__builtin_amdgcn_ballot_w64(smin < P.s_lo)) {  // rare: close pairs in some list
+                        for (int c = 0; c < nchunk; ++c) {
+                            const unsigned w = chunk_word(c);
+                            R dx, dy, dz;
+                            disp((int)(w & 0xffffu), dx, dy, dz);
+                            pp_pair_fix(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u);
+                            disp((int)(w >> 16), dx, dy, dz);
+                            pp_pair_fix(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u);
+                        }
                     }
                 }
             }
