@@ -369,6 +369,274 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const typename Cx<T>::t *__re
     c2r_body<T>(ek, emesh, m, lm, Mxh, np, nspec, tw, blockIdx.x, smraw);
 }
 
+// ---- register radix-8 transforms of strided lines (passes 2, 3, 4) --------
+// A line of m = 2^lm >= 8 elements is held by P = m/8 threads, eight
+// elements each: thread p holds elements p + kP, k = 0..7.  The transform runs
+// as Stockham stages of radix 8 (then one of radix 4 or 2 when 3 does not
+// divide lm).  A stage is one set of butterflies per thread on its registers
+// (radix 8: one; radix 4: two, on registers k even and odd; radix 2: four, on
+// registers c and c + 4); the stages exchange through LDS, where a block's KT
+// lines sit interleaved (element e of line l at e * KT + l, so that the reads
+// of p + kP are consecutive across the lines of a wave).  Every stage reads
+// positions p + kP, and the last stage's outputs land on those same
+// positions, so the first stage takes its input from registers and the last
+// leaves its output in them: two LDS exchanges for m = 512 against the six
+// reads and writes of the LDS Stockham transform, and one buffer of m KT
+// elements instead of two (16 columns a block at 512 against 4 there).
+// Stage of radix R and span Ns: butterfly j, k = j mod Ns, input x_q =
+// x[j + q m/R] turned by exp(DIR 2 pi i q k / (R Ns)), R-point DFT, output r
+// to (j - k) R + k + r Ns, as the LDS transform above.
+
+// exp(DIR 2 pi i t / m), 0 <= t < m, from the forward table of the first m/2
+template <int DIR, class C>
+__device__ __forceinline__ C twd(const C *__restrict__ tw, int t, int half) {
+    C w = tw[t < half ? t : t - half];
+    if (t >= half) {
+        w.x = -w.x;
+        w.y = -w.y;
+    }
+    if (DIR > 0) w.y = -w.y;
+    return w;
+}
+template <class C>
+__device__ __forceinline__ C cadd(C a, C b) {
+    a.x += b.x;
+    a.y += b.y;
+    return a;
+}
+template <class C>
+__device__ __forceinline__ C csub(C a, C b) {
+    a.x -= b.x;
+    a.y -= b.y;
+    return a;
+}
+// a times DIR i
+template <int DIR, class C>
+__device__ __forceinline__ C muli(C a) {
+    C r;
+    if (DIR < 0) {
+        r.x = a.y;
+        r.y = -a.x;
+    } else {
+        r.x = -a.y;
+        r.y = a.x;
+    }
+    return r;
+}
+// 4-point DFT in place, natural order
+template <int DIR, class C>
+__device__ __forceinline__ void dft4(C &x0, C &x1, C &x2, C &x3) {
+    const C c0 = cadd(x0, x2), c1 = csub(x0, x2), c2 = cadd(x1, x3), c3 = muli<DIR>(csub(x1, x3));
+    x0 = cadd(c0, c2);
+    x1 = cadd(c1, c3);
+    x2 = csub(c0, c2);
+    x3 = csub(c1, c3);
+}
+// 8-point DFT in place, natural order: even outputs the DFT4 of x_k + x_{k+4},
+// odd ones that of (x_k - x_{k+4}) exp(DIR 2 pi i k / 8)
+template <int DIR, class C>
+__device__ __forceinline__ void dft8(C *x) {
+    using T = decltype(x[0].x);
+    const T r2 = (T)0.70710678118654752440;
+    C a0 = cadd(x[0], x[4]), a1 = cadd(x[1], x[5]), a2 = cadd(x[2], x[6]), a3 = cadd(x[3], x[7]);
+    C b0 = csub(x[0], x[4]), b1 = csub(x[1], x[5]), b2 = csub(x[2], x[6]), b3 = csub(x[3], x[7]);
+    const C i1 = muli<DIR>(b1), i3 = muli<DIR>(b3);
+    b1.x = r2 * (b1.x + i1.x);
+    b1.y = r2 * (b1.y + i1.y);
+    b2 = muli<DIR>(b2);
+    b3.x = r2 * (i3.x - b3.x);
+    b3.y = r2 * (i3.y - b3.y);
+    dft4<DIR>(a0, a1, a2, a3);
+    dft4<DIR>(b0, b1, b2, b3);
+    x[0] = a0;
+    x[1] = b0;
+    x[2] = a1;
+    x[3] = b1;
+    x[4] = a2;
+    x[5] = b2;
+    x[6] = a3;
+    x[7] = b3;
+}
+
+// The transform of one line (all threads of the block call it: it holds
+// barriers).  v: this thread's elements p + kP, in and out.
+template <int DIR, class C>
+__device__ void reg_fft(C *v, int lm, int p, int l, int KT, C *lds, const C *__restrict__ tw) {
+    const int m = 1 << lm, P = m >> 3, half = m >> 1;
+    const int n8 = lm / 3, rem = lm - 3 * n8;
+    const int nst = n8 + (rem ? 1 : 0);
+    int s = 0;  // log2 of the span Ns
+    for (int st = 0; st < nst; ++st) {
+        const int R = st < n8 ? 8 : (rem == 2 ? 4 : 2);
+        const bool last = st == nst - 1;
+        const int Ns = 1 << s;
+        if (st > 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = lds[(p + k * P) * KT + l];
+        }
+        if (R == 8) {
+            const int j = p, k = j & (Ns - 1);
+            if (s > 0) {
+                const int t1 = k << (lm - 3 - s);
+                const C w1 = twd<DIR>(tw, t1, half), w2 = twd<DIR>(tw, 2 * t1, half), w4 = twd<DIR>(tw, 4 * t1, half);
+                const C w3 = cmul(w1, w2);
+                v[1] = cmul(v[1], w1);
+                v[2] = cmul(v[2], w2);
+                v[3] = cmul(v[3], w3);
+                v[4] = cmul(v[4], w4);
+                v[5] = cmul(v[5], cmul(w1, w4));
+                v[6] = cmul(v[6], cmul(w2, w4));
+                v[7] = cmul(v[7], cmul(w3, w4));
+            }
+            dft8<DIR>(v);
+            if (!last) {
+                __syncthreads();  // every thread has read this stage's inputs
+#pragma unroll
+                for (int r = 0; r < 8; ++r) lds[((j - k) * 8 + k + r * Ns) * KT + l] = v[r];
+                __syncthreads();
+            }
+        } else if (R == 4) {
+            int o[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int j = p + c * P, k = j & (Ns - 1);
+                o[c] = (j - k) * 4 + k;
+                if (s > 0) {
+                    const int t1 = k << (lm - 2 - s);
+                    const C w1 = twd<DIR>(tw, t1, half), w2 = twd<DIR>(tw, 2 * t1, half);
+                    v[2 + c] = cmul(v[2 + c], w1);
+                    v[4 + c] = cmul(v[4 + c], w2);
+                    v[6 + c] = cmul(v[6 + c], cmul(w1, w2));
+                }
+                dft4<DIR>(v[c], v[2 + c], v[4 + c], v[6 + c]);
+            }
+            if (!last) {
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) lds[(o[c] + r * Ns) * KT + l] = v[2 * r + c];
+                __syncthreads();
+            }
+        } else {
+            int o[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int j = p + c * P, k = j & (Ns - 1);
+                o[c] = (j - k) * 2 + k;
+                if (s > 0) v[c + 4] = cmul(v[c + 4], twd<DIR>(tw, k << (lm - 1 - s), half));
+                const C a = v[c], b = v[c + 4];
+                v[c] = cadd(a, b);
+                v[c + 4] = csub(a, b);
+            }
+            if (!last) {
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    lds[o[c] * KT + l] = v[c];
+                    lds[(o[c] + Ns) * KT + l] = v[c + 4];
+                }
+                __syncthreads();
+            }
+        }
+        s += R == 8 ? 3 : (R == 4 ? 2 : 1);
+    }
+}
+
+// Passes 2 and 4: lines along a strided axis.  Element e of line (o, kx) at
+// o*os + e*es + kx; a block takes KT consecutive kx of outer index o
+// (blockIdx.y) in field blockIdx.z; blockDim = KT * m/8 (at most 1024 in
+// FP32, 512 in FP64, whose registers would not fit 1024 threads).
+template <int DIR, class C>
+__global__ __launch_bounds__(sizeof(C) == 8 ? 1024 : 512) void k_fft_col(C *__restrict__ data, size_t fstride, int lm, int es, int os, int Mxh,
+                                                  int KT, const C *__restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    C *lds = (C *)smraw;
+    const int l = threadIdx.x % KT, p = threadIdx.x / KT;
+    const int P = (1 << lm) >> 3;
+    const int kx = blockIdx.x * KT + l;
+    const bool ok = kx < Mxh;
+    C *base = data + blockIdx.z * fstride + (size_t)blockIdx.y * os + (ok ? kx : 0);
+    C v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        v[k] = ok ? base[(size_t)(p + k * P) * es] : C{};
+    }
+    reg_fft<DIR>(v, lm, p, l, KT, lds, tw);
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) base[(size_t)(p + k * P) * es] = v[k];
+    }
+}
+
+// Pass 3 with register transforms: z forward, G, energy, three ik fields and
+// three inverse z transforms; a block takes KT consecutive kx of one y.
+template <class T>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fft_zgreen_r(GreenZArgs<T> P) {
+    using C = typename Cx<T>::t;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    __shared__ double red[16];
+    C *lds = (C *)smraw;
+    const int KT = P.kt, lm = P.lm;
+    const int l = threadIdx.x % KT, p = threadIdx.x / KT;
+    const int Pn = (1 << lm) >> 3;
+    const int y = blockIdx.y;
+    const int kx = blockIdx.x * KT + l;
+    const bool ok = kx < P.Mxh;
+    const size_t es = (size_t)P.M[1] * P.Mxh;
+    const size_t off = (size_t)y * P.Mxh + (ok ? kx : 0);
+    C v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = ok ? P.ck[off + (size_t)(p + k * Pn) * es] : C{};
+    reg_fft<-1>(v, lm, p, l, KT, lds, P.tw);
+    double esum = 0.0;
+    const T inv_V = (T)P.inv_V;
+    const double wgt = (kx == 0 || 2 * kx == P.M[0]) ? 1.0 : 2.0;
+    C phi[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int kz = p + k * Pn;
+        const T gk = ok ? P.green[off + (size_t)kz * es] : (T)0;
+        esum += wgt * (double)gk * ((double)v[k].x * (double)v[k].x + (double)v[k].y * (double)v[k].y);
+        phi[k] = Cx<T>::make(gk * v[k].x * inv_V, gk * v[k].y * inv_V);
+    }
+    esum = wave_sum(esum);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = esum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += red[w];
+        P.part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        const int Mn = P.M[ax];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int kz = p + k * Pn;
+            const int n = ax == 0 ? kx : (ax == 1 ? y : kz);
+            const T kk = (T)((2 * n == Mn) ? 0.0 : kval(n, Mn, P.kfac[ax]));
+            v[k] = Cx<T>::make(kk * phi[k].y, -kk * phi[k].x);  // -i k phi
+        }
+        reg_fft<1>(v, lm, p, l, KT, lds, P.tw);
+        if (ok) {
+            C *dst = P.ek + ax * P.nspec + off;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) dst[(size_t)(p + k * Pn) * es] = v[k];
+        }
+    }
+}
+
+// Lines a block of the register transforms: 16 (or as many as make one full
+// wave of m/8 threads a line, for m < 32), fewer when the block would exceed
+// 1024 threads (FP32) or 512 (FP64) or the LDS buffer (m KT elements) 64 KB;
+// the block stays a whole number of waves.
+static int reg_lines(int m, size_t cs) {
+    const int maxt = cs == 8 ? 1024 : 512, P = m / 8;
+    int kt = P >= 4 ? 16 : 64 / P;
+    while (kt * P > 64 && (kt * P > maxt || (size_t)m * kt * cs > 64 * 1024)) kt >>= 1;
+    return kt;
+}
+
 static int ilog2(int m) {
     int l = 0;
     while ((1 << l) < m) ++l;
@@ -465,14 +733,39 @@ static int pm_solve_t(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const typena
     const FftPlan<T> F = fft_plan<T>(c, tw0, tw1, tw2, ck, ek, green, emesh);
     hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(F.g1), dim3(256), F.lds1, s, c->rho, ck, F.Mx, F.lx, F.Mxh, F.nl, tw0);
     MDF_LAUNCHED();
-    hipLaunchKernelGGL((k_fft_strided<-1, C>), dim3(F.g2x, F.Mz, 1), dim3(256), F.lds2, s, ck, (size_t)0, F.My,
-                       F.ly, F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
+    // y and z: register transforms (lines of 8 or more elements)
+    const bool reg_y = F.ly >= 3, reg_z = F.lz >= 3;
+    const int kty = reg_lines(F.My, sizeof(C)), ktz = reg_lines(F.Mz, sizeof(C));
+    const int gy = (F.Mxh + kty - 1) / kty, gz = (F.Mxh + ktz - 1) / ktz;
+    const size_t ldsy = (size_t)F.My * kty * sizeof(C), ldsz = (size_t)F.Mz * ktz * sizeof(C);
+    if (reg_y) {
+        MDF_TRY(allow_lds(k_fft_col<-1, C>, ldsy));
+        hipLaunchKernelGGL((k_fft_col<-1, C>), dim3(gy, F.Mz, 1), dim3(kty * (F.My / 8)), ldsy, s, ck, (size_t)0,
+                           F.ly, F.Mxh, F.My * F.Mxh, F.Mxh, kty, tw1);
+    } else {
+        hipLaunchKernelGGL((k_fft_strided<-1, C>), dim3(F.g2x, F.Mz, 1), dim3(256), F.lds2, s, ck, (size_t)0, F.My,
+                           F.ly, F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
+    }
     MDF_LAUNCHED();
-    c->n_part_pm = F.g3x * F.My;
-    hipLaunchKernelGGL(k_fft_z_green<T>, dim3(F.g3x, F.My), dim3(256), F.lds3, s, F.gz);
+    if (reg_z) {
+        GreenZArgs<T> G = F.gz;
+        G.kt = ktz;
+        c->n_part_pm = gz * F.My;
+        MDF_TRY(allow_lds(k_fft_zgreen_r<T>, ldsz));
+        hipLaunchKernelGGL(k_fft_zgreen_r<T>, dim3(gz, F.My), dim3(ktz * (F.Mz / 8)), ldsz, s, G);
+    } else {
+        c->n_part_pm = F.g3x * F.My;
+        hipLaunchKernelGGL(k_fft_z_green<T>, dim3(F.g3x, F.My), dim3(256), F.lds3, s, F.gz);
+    }
     MDF_LAUNCHED();
-    hipLaunchKernelGGL((k_fft_strided<1, C>), dim3(F.g2x, F.Mz, 3), dim3(256), F.lds2, s, ek, c->nspec, F.My, F.ly,
-                       F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
+    if (reg_y) {
+        MDF_TRY(allow_lds(k_fft_col<1, C>, ldsy));
+        hipLaunchKernelGGL((k_fft_col<1, C>), dim3(gy, F.Mz, 3), dim3(kty * (F.My / 8)), ldsy, s, ek, c->nspec, F.ly,
+                           F.Mxh, F.My * F.Mxh, F.Mxh, kty, tw1);
+    } else {
+        hipLaunchKernelGGL((k_fft_strided<1, C>), dim3(F.g2x, F.Mz, 3), dim3(256), F.lds2, s, ek, c->nspec, F.My,
+                           F.ly, F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
+    }
     MDF_LAUNCHED();
     hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(F.g5), dim3(256), F.lds5, s, ek, emesh, F.Mx, F.lx, F.Mxh, F.np,
                        c->nspec, tw0);
