@@ -458,10 +458,24 @@ __device__ __forceinline__ void dft8(C *x) {
     x[7] = b3;
 }
 
+// LDS index of element e of a thread's line: interleaved lines (e * KT + l)
+// for the column passes, whose threads run line-fastest, and consecutive
+// lines of stride m + 16 for the row passes, whose threads run
+// element-fastest (the 16-element pad puts the lines of one wave on
+// different banks).
+struct ColIdx {
+    int l, KT;
+    __device__ __forceinline__ int operator()(int e) const { return e * KT + l; }
+};
+struct RowIdx {
+    int base;
+    __device__ __forceinline__ int operator()(int e) const { return base + e; }
+};
+
 // The transform of one line (all threads of the block call it: it holds
 // barriers).  v: this thread's elements p + kP, in and out.
-template <int DIR, class C>
-__device__ void reg_fft(C *v, int lm, int p, int l, int KT, C *lds, const C *__restrict__ tw) {
+template <int DIR, class C, class Idx>
+__device__ void reg_fft(C *v, int lm, int p, Idx at, C *lds, const C *__restrict__ tw) {
     const int m = 1 << lm, P = m >> 3, half = m >> 1;
     const int n8 = lm / 3, rem = lm - 3 * n8;
     const int nst = n8 + (rem ? 1 : 0);
@@ -472,7 +486,7 @@ __device__ void reg_fft(C *v, int lm, int p, int l, int KT, C *lds, const C *__r
         const int Ns = 1 << s;
         if (st > 0) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) v[k] = lds[(p + k * P) * KT + l];
+            for (int k = 0; k < 8; ++k) v[k] = lds[at(p + k * P)];
         }
         if (R == 8) {
             const int j = p, k = j & (Ns - 1);
@@ -492,7 +506,7 @@ __device__ void reg_fft(C *v, int lm, int p, int l, int KT, C *lds, const C *__r
             if (!last) {
                 __syncthreads();  // every thread has read this stage's inputs
 #pragma unroll
-                for (int r = 0; r < 8; ++r) lds[((j - k) * 8 + k + r * Ns) * KT + l] = v[r];
+                for (int r = 0; r < 8; ++r) lds[at((j - k) * 8 + k + r * Ns)] = v[r];
                 __syncthreads();
             }
         } else if (R == 4) {
@@ -515,7 +529,7 @@ __device__ void reg_fft(C *v, int lm, int p, int l, int KT, C *lds, const C *__r
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) lds[(o[c] + r * Ns) * KT + l] = v[2 * r + c];
+                    for (int r = 0; r < 4; ++r) lds[at(o[c] + r * Ns)] = v[2 * r + c];
                 __syncthreads();
             }
         } else {
@@ -533,8 +547,8 @@ __device__ void reg_fft(C *v, int lm, int p, int l, int KT, C *lds, const C *__r
                 __syncthreads();
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    lds[o[c] * KT + l] = v[c];
-                    lds[(o[c] + Ns) * KT + l] = v[c + 4];
+                    lds[at(o[c])] = v[c];
+                    lds[at(o[c] + Ns)] = v[c + 4];
                 }
                 __syncthreads();
             }
@@ -562,7 +576,7 @@ __global__ __launch_bounds__(sizeof(C) == 8 ? 1024 : 512) void k_fft_col(C *__re
     for (int k = 0; k < 8; ++k) {
         v[k] = ok ? base[(size_t)(p + k * P) * es] : C{};
     }
-    reg_fft<DIR>(v, lm, p, l, KT, lds, tw);
+    reg_fft<DIR>(v, lm, p, ColIdx{l, KT}, lds, tw);
     if (ok) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) base[(size_t)(p + k * P) * es] = v[k];
@@ -588,7 +602,7 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fft_zgreen_r(Gr
     C v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = ok ? P.ck[off + (size_t)(p + k * Pn) * es] : C{};
-    reg_fft<-1>(v, lm, p, l, KT, lds, P.tw);
+    reg_fft<-1>(v, lm, p, ColIdx{l, KT}, lds, P.tw);
     double esum = 0.0;
     const T inv_V = (T)P.inv_V;
     const double wgt = (kx == 0 || 2 * kx == P.M[0]) ? 1.0 : 2.0;
@@ -617,13 +631,113 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fft_zgreen_r(Gr
             const T kk = (T)((2 * n == Mn) ? 0.0 : kval(n, Mn, P.kfac[ax]));
             v[k] = Cx<T>::make(kk * phi[k].y, -kk * phi[k].x);  // -i k phi
         }
-        reg_fft<1>(v, lm, p, l, KT, lds, P.tw);
+        reg_fft<1>(v, lm, p, ColIdx{l, KT}, lds, P.tw);
         if (ok) {
             C *dst = P.ek + ax * P.nspec + off;
 #pragma unroll
             for (int k = 0; k < 8; ++k) dst[(size_t)(p + k * Pn) * es] = v[k];
         }
     }
+}
+
+// Pass 1 with register transforms: rows 2q, 2q+1 of the FP64 charge mesh
+// as one complex line (blockDim = KT * m/8, line-major threads), forward,
+// then through LDS to the Hermitian split of the two rows' half spectra.
+template <class T>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fft_row_r2c(const double *__restrict__ rho,
+                                                                                typename Cx<T>::t *__restrict__ ck,
+                                                                                int lm, int Mxh, int KT, int npair,
+                                                                                const typename Cx<T>::t *__restrict__ tw) {
+    using C = typename Cx<T>::t;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    C *lds = (C *)smraw;
+    const int m = 1 << lm, P = m >> 3;
+    const int l = threadIdx.x / P, p = threadIdx.x - l * P;
+    const int q = blockIdx.x * KT + l;
+    const bool ok = q < npair;
+    const size_t ra = 2 * (size_t)(ok ? q : 0);
+    C v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int x = p + k * P;
+        v[k] = ok ? Cx<T>::make((T)rho[ra * m + x], (T)rho[(ra + 1) * m + x]) : C{};
+    }
+    const RowIdx at{l * (m + 16)};
+    reg_fft<-1>(v, lm, p, at, lds, tw);
+    __syncthreads();  // every thread has read the last stage's inputs
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lds[at(p + k * P)] = v[k];
+    __syncthreads();
+    if (!ok) return;
+    for (int k = p; k < Mxh; k += P) {
+        const C zk = lds[at(k)];
+        const C zc = lds[at((m - k) & (m - 1))];
+        // A = (Z_k + conj Z_{m-k})/2, B = (Z_k - conj Z_{m-k})/(2i)
+        ck[ra * Mxh + k] = Cx<T>::make((T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y - zc.y));
+        ck[(ra + 1) * Mxh + k] = Cx<T>::make((T)0.5 * (zk.y + zc.y), (T)-0.5 * (zk.x - zc.x));
+    }
+}
+
+// Pass 5 with register transforms: per row pair (R, R+1) three complex lines
+// (Ex_R + i Ey_R, Ex_R+1 + i Ey_R+1, Ez_R + i Ez_R+1), each element read
+// straight from the half spectra, inverse, written to the interleaved
+// (Ex, Ey, Ez) mesh: the line's real and imaginary parts go to their
+// components (blockDim = 3 NP * m/8, line-major threads).
+template <class T>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fft_row_c2r(const typename Cx<T>::t *__restrict__ ek,
+                                                                                T *__restrict__ emesh, int lm, int Mxh,
+                                                                                int NP, int npair, size_t nspec,
+                                                                                const typename Cx<T>::t *__restrict__ tw) {
+    using C = typename Cx<T>::t;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    C *lds = (C *)smraw;
+    const int m = 1 << lm, P = m >> 3;
+    const int L = threadIdx.x / P, p = threadIdx.x - L * P;
+    const int pr = L / 3, f = L - 3 * pr;  // row pair in the block, line kind
+    const int q = blockIdx.x * NP + pr;
+    const bool ok = q < npair;
+    const size_t R = 2 * (size_t)(ok ? q : 0);
+    // line f = 0: Ex_R + i Ey_R; 1: Ex_R+1 + i Ey_R+1; 2: Ez_R + i Ez_R+1
+    const C *a = f < 2 ? ek + (R + f) * Mxh : ek + 2 * nspec + R * Mxh;
+    const C *b = f < 2 ? ek + nspec + (R + f) * Mxh : ek + 2 * nspec + (R + 1) * Mxh;
+    C v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int x = p + k * P;
+        if (ok) {
+            const C X = herm_get(a, x, m), Y = herm_get(b, x, m);
+            v[k] = Cx<T>::make(X.x - Y.y, X.y + Y.x);
+        } else {
+            v[k] = C{};
+        }
+    }
+    reg_fft<1>(v, lm, p, RowIdx{L * (m + 16)}, lds, tw);
+    if (!ok) return;
+    T *n0 = emesh + 3 * (R * m), *n1 = emesh + 3 * ((R + 1) * m);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int x = p + k * P;
+        if (f == 0) {
+            n0[3 * x + 0] = v[k].x;
+            n0[3 * x + 1] = v[k].y;
+        } else if (f == 1) {
+            n1[3 * x + 0] = v[k].x;
+            n1[3 * x + 1] = v[k].y;
+        } else {
+            n0[3 * x + 2] = v[k].x;
+            n1[3 * x + 2] = v[k].y;
+        }
+    }
+}
+
+// Row pairs a block of the row passes: lines of m/8 threads and m + 16 LDS
+// elements, at most 1024 threads (FP32) or 512 (FP64) and 64 KB; lpp lines a
+// pair (1 in pass 1, 3 in pass 5).
+static int row_pairs(int m, size_t cs, int lpp) {
+    const int maxt = cs == 8 ? 1024 : 512, P = m / 8;
+    int np = 16;
+    while (np > 1 && (np * lpp * P > maxt || (size_t)np * lpp * (m + 16) * cs > 64 * 1024)) --np;
+    return np;
 }
 
 // Lines a block of the register transforms: 16 (or as many as make one full
@@ -731,7 +845,19 @@ static int pm_solve_t(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const typena
                       T *emesh, hipStream_t s) {
     using C = typename Cx<T>::t;
     const FftPlan<T> F = fft_plan<T>(c, tw0, tw1, tw2, ck, ek, green, emesh);
-    hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(F.g1), dim3(256), F.lds1, s, c->rho, ck, F.Mx, F.lx, F.Mxh, F.nl, tw0);
+    // x: register transforms of row pairs (rows of 8 or more elements)
+    const bool reg_x = F.lx >= 3;
+    const int npair = F.My * F.Mz / 2, P8 = F.Mx / 8;
+    if (reg_x) {
+        const int np1 = row_pairs(F.Mx, sizeof(C), 1);
+        const size_t lds = (size_t)np1 * (F.Mx + 16) * sizeof(C);
+        MDF_TRY(allow_lds(k_fft_row_r2c<T>, lds));
+        hipLaunchKernelGGL(k_fft_row_r2c<T>, dim3((npair + np1 - 1) / np1), dim3(np1 * P8), lds, s, c->rho, ck,
+                           F.lx, F.Mxh, np1, npair, tw0);
+    } else {
+        hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(F.g1), dim3(256), F.lds1, s, c->rho, ck, F.Mx, F.lx, F.Mxh, F.nl,
+                           tw0);
+    }
     MDF_LAUNCHED();
     // y and z: register transforms (lines of 8 or more elements)
     const bool reg_y = F.ly >= 3, reg_z = F.lz >= 3;
@@ -767,8 +893,16 @@ static int pm_solve_t(mdf_p3m_ctx *c, const typename Cx<T>::t *tw0, const typena
                            F.ly, F.Mxh, F.My * F.Mxh, F.Mxh, F.kty, tw1);
     }
     MDF_LAUNCHED();
-    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(F.g5), dim3(256), F.lds5, s, ek, emesh, F.Mx, F.lx, F.Mxh, F.np,
-                       c->nspec, tw0);
+    if (reg_x) {
+        const int np5 = row_pairs(F.Mx, sizeof(C), 3);
+        const size_t lds = (size_t)np5 * 3 * (F.Mx + 16) * sizeof(C);
+        MDF_TRY(allow_lds(k_fft_row_c2r<T>, lds));
+        hipLaunchKernelGGL(k_fft_row_c2r<T>, dim3((npair + np5 - 1) / np5), dim3(np5 * 3 * P8), lds, s, ek, emesh,
+                           F.lx, F.Mxh, np5, npair, c->nspec, tw0);
+    } else {
+        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(F.g5), dim3(256), F.lds5, s, ek, emesh, F.Mx, F.lx, F.Mxh, F.np,
+                           c->nspec, tw0);
+    }
     MDF_LAUNCHED();
     return MDF_OK;
 }
