@@ -386,26 +386,38 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], do
             // of a CU (at 144 KB the FP64 brick of 2^3 cells at N = 1e6, rc 7,
             // 152 KB, fell back to 1-cell blocks: 0.45 ms against 0.27 ms).
             // When no shape fits, 2^3 sub-cells read the global field meshes.
+            // The FP32 brick's planes are padded (gather_plane_f32); the LDS
+            // left beside it holds the particles' visit order, 2 B each (up to
+            // 4096, four a thread; a block with more keeps the sorted order).
+            auto brick_bytes = [&](const int *sb, int *B) {
+                extent(sb, B);
+                return gather_brick_bytes(B, mixed);
+            };
             int k = 0;
             int B[3];
-            while (k < 7 && extent(shapes[k], B) * 3 * es > 156 * 1024) ++k;
+            while (k < 7 && brick_bytes(shapes[k], B) > 156 * 1024) ++k;
             bool fits = k < 7;
             if (!fits) k = 3;
             if (req[2]) {
                 k = req[2] - 1;
-                if (extent(shapes[k], B) * 3 * es > 156 * 1024) {
+                if (brick_bytes(shapes[k], B) > 156 * 1024) {
                     delete c;
                     set_error("gather block shape %d: its field brick exceeds the 156 KB of LDS", req[2]);
                     return MDF_EINVAL;
                 }
                 fits = true;
             }
-            extent(shapes[k], B);
+            const size_t bb = brick_bytes(shapes[k], B);
             for (int d = 0; d < 3; ++d) {
                 c->brick.sb_gather[d] = shapes[k][d];
                 c->brick.Bg[d] = B[d];
             }
             c->brick.lds_gather = fits ? 1 : 0;
+            c->brick.ord_cap = 0;
+            if (fits && mixed && bb < 156 * 1024) {
+                const size_t cap = (156 * 1024 - bb) / 2;
+                c->brick.ord_cap = (int)(cap < 4 * (size_t)kGatherThreads ? cap : 4 * (size_t)kGatherThreads);
+            }
             c->shape_gather = k + 1;
         }
         c->brick.nblock_assign = nblocks(c->brick.sb_assign);

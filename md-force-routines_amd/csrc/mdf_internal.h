@@ -64,7 +64,30 @@ struct Brick {
     int nblock_assign, nblock_gather;
     int lds_assign;    // 1 if the charge brick fits LDS for the assignment
     int lds_gather;    // 1 if the 3-field brick fits LDS for the gather
+    int ord_cap;       // FP32 gather: particles a block the bank-spread visit order holds (0: none)
 };
+
+// FP32 gather brick (Ex, Ey, Ez a node): node (x, y, z) at float 3 x + Sy y +
+// Sz z, Sy = 3 B0 and Sz the 3 B0 B1 floats of a plane rounded up to 16 (mod
+// 32).  The gather's LDS reads behave as 32 banks of 4 bytes served a
+// half-wave at a time (its bank-conflict counters follow that model to within
+// 5%); a particle's mirrored plane pairs (offsets 5, 1 and -3 planes) are then
+// 16 banks apart and its two x halves (nodes 0-2 and 3-5) 9 banks apart, so
+// the four lanes of a particle hit four banks whose set is fixed by its base
+// address mod 16.  The kernel visits a block's particles in an order that puts
+// 8 particles of distinct base (mod 16, one parity) in each half-wave, which
+// makes a half-wave's reads conflict-free.
+__host__ __device__ inline int gather_plane_f32(int B0, int B1) {
+    const int s = 3 * B0 * B1;
+    return s + ((16 - s) & 31);
+}
+// dynamic LDS of the gather: the field brick, plus (FP32) the visit order
+inline size_t gather_brick_bytes(const int *Bg, bool mixed) {
+    return mixed ? 4 * (size_t)gather_plane_f32(Bg[0], Bg[1]) * Bg[2] : 24 * (size_t)Bg[0] * Bg[1] * Bg[2];
+}
+inline size_t gather_lds_bytes(const Brick &b, bool mixed) {
+    return gather_brick_bytes(b.Bg, mixed) + (mixed ? 2 * (size_t)b.ord_cap : 0);
+}
 
 // FP32 pair table indexed by the float bits of s' = sigma^2 r^2, the squared
 // distance in the mixed PP kernel's tile units 1 / sigma (pair_table.hip): piece

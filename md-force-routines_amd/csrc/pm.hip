@@ -10,8 +10,9 @@
 // context so the block's mesh brick fits LDS; the block's particles are
 // contiguous runs of the sorted array).  Each particle gets a group of
 // lanes: eight for the scatter, six of them taking one z-plane (36 nodes) of
-// its 6x6x6 stencil each; four for the gather, each taking a quarter (three
-// z-planes by three y-rows of six x nodes):
+// its 6x6x6 stencil each; four for the gather, each taking a quarter (FP64:
+// three z-planes by three y-rows of six x nodes; FP32: three z-planes by six
+// y-rows of three x nodes):
 // * scatter: the block's particles deposit into a private LDS brick of the
 //   mesh with LDS atomics, then the brick is added to the global mesh with
 //   FP64 atomics (one per non-empty brick node instead of 216 per particle);
@@ -60,6 +61,17 @@ template <class T>
 __device__ __forceinline__ T w6_2(T x) {
     return ((T)841 + x * ((T)-770 + x * ((T)-440 + x * ((T)560 + x * ((T)80 + x * (T)-160))))) * WNorm<T>::c2;
 }
+// w0, w1 or w2 at x by k (0, 1, 2) with the coefficients selected, for a k
+// that differs between lanes: one polynomial instead of three
+template <class T>
+__device__ __forceinline__ T w6_sel(int k, T x) {
+    auto c = [k](T a, T b, T d) { return k == 0 ? a : (k == 1 ? b : d); };
+    const T p = c((T)1, (T)237, (T)841) +
+                x * (c((T)-10, (T)-750, (T)-770) +
+                     x * (c((T)40, (T)840, (T)-440) +
+                          x * (c((T)-80, (T)-240, (T)560) + x * (c((T)80, (T)-240, (T)80) + x * c((T)-32, (T)160, (T)-160)))));
+    return p * (k == 2 ? WNorm<T>::c2 : WNorm<T>::c01);
+}
 template <class T>
 __device__ __forceinline__ void weights6(T x, T w[6]) {
     w[0] = w6_0<T>(x);
@@ -84,6 +96,24 @@ __device__ __forceinline__ int base_and_offset(double x, double ih, T &off) {
     return i0;
 }
 
+// Brick node index t = x + B0 (y + B1 z) split into (x, y, z) by two 32-bit
+// multiply-highs: m = ceil(2^32 / B) gives floor(t / B) exactly while t B <
+// 2^32 (bricks hold < 2^20 nodes of extent < 2^8); the integer divisions by
+// runtime extents cost ~15 instructions each.
+struct Split3 {
+    unsigned m0, m1;
+    int B0, B1;
+};
+__device__ __forceinline__ Split3 split3_init(int B0, int B1) {
+    return Split3{0xffffffffu / (unsigned)B0 + 1u, 0xffffffffu / (unsigned)B1 + 1u, B0, B1};
+}
+__device__ __forceinline__ void split3(const Split3 &d, int t, int &x, int &y, int &z) {
+    const int q = (int)__umulhi((unsigned)t, d.m0);
+    x = t - q * d.B0;
+    z = (int)__umulhi((unsigned)q, d.m1);
+    y = q - z * d.B1;
+}
+
 // Wrap for indices in [-M, 3M) (brick nodes and stencils, since M >= 8 and a
 // brick spans at most M/3 + 9 nodes).
 __device__ __forceinline__ int wrap3(int i, int M) {
@@ -93,7 +123,7 @@ __device__ __forceinline__ int wrap3(int i, int M) {
     return i;
 }
 
-// Gather: a particle's stencil is split over four consecutive lanes: lane
+// FP64 gather: a particle's stencil is split over four consecutive lanes: lane
 // bit 0 picks the y rows (0, 1, 2 or 5, 4, 3), bit 1 the z planes (0, 2, 4 or
 // 5, 3, 1).  By the mirror identity above the weights of a lane's rows are
 // w0, w1, w2 at +-y and of its planes w0, w2 at +-z and w1 at -+z: every lane
@@ -215,6 +245,13 @@ __device__ __forceinline__ int pm_particle(const int *run_start, const int *run_
     return run_start[lo] + (k - run_off[lo]);
 }
 
+// The same for a k that never decreases between calls of one thread: the run
+// index advances from the last call's (empty runs are stepped over).
+__device__ __forceinline__ int pm_particle_seq(const int *run_start, const int *run_off, int nrun, int k, int &run) {
+    while (run + 1 < nrun && run_off[run + 1] <= k) ++run;
+    return run_start[run] + (k - run_off[run]);
+}
+
 struct AssignArgs {
     Geom g;
     Brick br;
@@ -247,6 +284,7 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     const Geom &g = P.g;
     const int B0 = LDS ? P.br.B[0] : 0, B1 = LDS ? P.br.B[1] : 0, B2 = LDS ? P.br.B[2] : 0;
     const int nb = B0 * B1 * B2;
+    const Split3 sp3 = split3_init(max(B0, 1), max(B1, 1));
     const PMBlock blk = pm_block(g, P.br.sb_assign, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     for (int t = threadIdx.x; t < nb; t += blockDim.x) {
@@ -260,8 +298,9 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     if constexpr (kFix) {
         __shared__ float zsum_w[kPMThreads / 64];
         float zs = 0.0f;
+        int run = 0;
         for (int k = threadIdx.x; k < npart; k += blockDim.x)
-            zs += fabsf((float)P.spos[pm_particle(run_start, run_off, nrun, k)].w);
+            zs += fabsf((float)P.spos[pm_particle_seq(run_start, run_off, nrun, k, run)].w);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) zs += __shfl_xor(zs, o, 64);
         if ((threadIdx.x & 63) == 0) zsum_w[threadIdx.x >> 6] = zs;
@@ -282,24 +321,35 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     // LDS atomics of one instruction land on the same node (measured ~6%
     // faster here; the gather, with plain reads, gains from the denser split)
     const int sub = threadIdx.x % kScatterLanes;
-    for (int k = threadIdx.x / kScatterLanes; k < npart; k += blockDim.x / kScatterLanes) {
+    int run = 0;
+    // each particle's position is loaded one trip ahead, so its memory
+    // latency overlaps the deposits of the particle before
+    // (unconditionally: a trip past the end reads the block's last particle)
+    const int kstep = blockDim.x / kScatterLanes;
+    int k = threadIdx.x / kScatterLanes;
+    int si_n = 0;
+    double4 p_n = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (k < npart) {
+        si_n = pm_particle_seq(run_start, run_off, nrun, k, run);
+        p_n = P.spos[si_n];
+    }
+    for (; k < npart; k += kstep) {
+        const int si = si_n;
+        const double4 p = p_n;
+        si_n = pm_particle_seq(run_start, run_off, nrun, min(k + kstep, npart - 1), run);
+        p_n = P.spos[si_n];
         if (sub >= 6) continue;
-        const int si = pm_particle(run_start, run_off, nrun, k);
-        const double4 p = P.spos[si];
-        T fx, fy, fz, wx[6], wy[6], wz[6];
+        T fx, fy, fz, wx[6], wy[6];
         const int ix = base_and_offset<T>(p.x, g.ih[0], fx);
         const int iy = base_and_offset<T>(p.y, g.ih[1], fy);
         const int iz = base_and_offset<T>(p.z, g.ih[2], fz);
         weights6<T>(fx, wx);
         weights6<T>(fy, wy);
-        weights6<T>(fz, wz);
         const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
         const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
-        T wza = wz[0];
-#pragma unroll
-        for (int a = 1; a < 6; ++a)
-            if (sub == a) wza = wz[a];
-        const T zw = (T)p.w * wza;
+        // this lane's plane weight w_sub(z) = w_{5-sub}(-z): one polynomial
+        // with lane-selected coefficients
+        const T zw = (T)p.w * w6_sel<T>(sub < 3 ? sub : 5 - sub, sub < 3 ? fz : -fz);
         if (kFix && inside) {
             T wxk[6];
 #pragma unroll
@@ -314,7 +364,7 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     const float y = fmaf(zwy, wxk[c], carry);
-                    const float f = floorf(y + 0.5f);
+                    const float f = rintf(y);  // nearest (ties to even): one instruction
                     carry = y - f;
                     atomicAdd(row + c, (int)f);
                 }
@@ -345,7 +395,8 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
     for (int t = threadIdx.x; t < nb; t += blockDim.x) {
         const double v = kFix ? (double)ibrick[t] * inv_s : brick[t] * wscale<T>();
         if (v != 0.0) {
-            const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
+            int tx, ty, tz;
+            split3(sp3, t, tx, ty, tz);
             const int gx = wrap3(o0 + tx, g.M[0]);
             const int gy = wrap3(o1 + ty, g.M[1]);
             const int gz = wrap3(o2 + tz, g.M[2]);
@@ -421,6 +472,37 @@ struct GatherArgs {
     GatherOut o;
 };
 
+// The gather's epilogue, split over the lanes of a particle: lane c < 3 of its
+// four (each holding the particle's summed field) writes component c of E_p and
+// of what the outputs ask for, in the caller's order i, with the fused second
+// half-kick; pp is component c of the particle's PP acceleration (read when
+// the outputs ask for it).  Returns that component's v^2 when kicking (else 0).
+template <class T>
+__device__ __forceinline__ double gather_out_c(const GatherArgs &P, int c, int i, double zq, double pp, T ex, T ey,
+                                               T ez) {
+    const T e = c == 0 ? ex : (c == 1 ? ey : ez);
+    const double ed = (double)e * wscale<T>();
+    const double ap = zq * ed;
+    const GatherOut &o = P.o;
+    double *eo = c == 0 ? o.ex : (c == 1 ? o.ey : o.ez);
+    if (eo) eo[i] = ed;
+    const size_t ic = 3 * (size_t)i + c;
+    if (o.acc_pm) o.acc_pm[ic] = ap;
+    if (o.acc_pp) o.acc_pp[ic] = pp;
+    double k2 = 0.0;
+    if (o.acc) {
+        const double a = o.include_pp ? (o.af * (pp + ap)) : ap;
+        o.acc[ic] = a;
+        if (o.vel) {
+#pragma clang fp contract(off)
+            const double v = o.vel[ic] + (0.5 * a) * o.kick_dt;
+            o.vel[ic] = v;
+            k2 = v * v;
+        }
+    }
+    return k2;
+}
+
 template <bool LDS, class T>
 __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, unsigned char *eraw, double *red,
                                             int *run_start, int *run_off) {
@@ -432,8 +514,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     if (LDS) {
         const int nb = B0 * B1 * B2;
+        const Split3 sp3 = split3_init(B0, B1);
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-            const int tx = t % B0, ty = (t / B0) % B1, tz = t / (B0 * B1);
+            int tx, ty, tz;
+            split3(sp3, t, tx, ty, tz);
             const size_t node = ((size_t)wrap3(o2 + tz, g.M[2]) * g.M[1] + wrap3(o1 + ty, g.M[1])) * g.M[0] +
                                 wrap3(o0 + tx, g.M[0]);
             ebrick[3 * t + 0] = emesh[3 * node + 0];
@@ -501,56 +585,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
         ex = quad_sum(ex);
         ey = quad_sum(ey);
         ez = quad_sum(ez);
-        if (!live || sub != 0) continue;
-        const int i = P.perm[s];
-        const double exd = (double)ex * wscale<T>(), eyd = (double)ey * wscale<T>(), ezd = (double)ez * wscale<T>();
-        const double apx = p.w * exd, apy = p.w * eyd, apz = p.w * ezd;
-        const GatherOut &o = P.o;
-        if (o.ex) o.ex[i] = exd;
-        if (o.ey) o.ey[i] = eyd;
-        if (o.ez) o.ez[i] = ezd;
-        if (o.acc_pm) {
-            o.acc_pm[3 * (size_t)i + 0] = apx;
-            o.acc_pm[3 * (size_t)i + 1] = apy;
-            o.acc_pm[3 * (size_t)i + 2] = apz;
-        }
-        double ppx = 0.0, ppy = 0.0, ppz = 0.0;
-        if (o.include_pp || o.acc_pp) {
-            ppx = P.acc_pp_s[3 * (size_t)s + 0];
-            ppy = P.acc_pp_s[3 * (size_t)s + 1];
-            ppz = P.acc_pp_s[3 * (size_t)s + 2];
-            if (o.acc_pp) {
-                o.acc_pp[3 * (size_t)i + 0] = ppx;
-                o.acc_pp[3 * (size_t)i + 1] = ppy;
-                o.acc_pp[3 * (size_t)i + 2] = ppz;
-            }
-        }
-        if (o.acc) {
-            double ax, ay, az;
-            if (o.include_pp) {
-                ax = (o.af * (ppx + apx));
-                ay = (o.af * (ppy + apy));
-                az = (o.af * (ppz + apz));
-            } else {
-                ax = apx;
-                ay = apy;
-                az = apz;
-            }
-            o.acc[3 * (size_t)i + 0] = ax;
-            o.acc[3 * (size_t)i + 1] = ay;
-            o.acc[3 * (size_t)i + 2] = az;
-            if (o.vel) {
-#pragma clang fp contract(off)
-                double *v = o.vel + 3 * (size_t)i;
-                const double vx = v[0] + (0.5 * ax) * o.kick_dt;
-                const double vy = v[1] + (0.5 * ay) * o.kick_dt;
-                const double vz = v[2] + (0.5 * az) * o.kick_dt;
-                v[0] = vx;
-                v[1] = vy;
-                v[2] = vz;
-                ksum += vx * vx + vy * vy + vz * vz;
-            }
-        }
+        if (!live || sub > 2) continue;
+        const double pp = (P.o.include_pp || P.o.acc_pp) ? P.acc_pp_s[3 * (size_t)s + sub] : 0.0;
+        ksum += gather_out_c<T>(P, sub, P.perm[s], p.w, pp, ex, ey, ez);
     }
     if (P.o.vel) {
         ksum = wave_sum(ksum);
@@ -564,12 +601,248 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
     }
 }
 
+// FP32 gather from the LDS field brick (layout at gather_plane_f32).  Four
+// lanes a particle: lane bit 0 takes the x nodes 0-2 or 3-5 of every row, bit 1
+// the z planes 0, 2, 4 or 5, 3, 1; each lane covers all six y rows of its
+// three planes.  By the mirror identity w[5-k](x) = w[k](-x) every lane
+// evaluates the same polynomials at +-x, y, -y and +-z.  The lanes of a
+// particle read at its base address plus 0, 9, 16 and 25 banks (mod 32), so a
+// half-wave whose 8 particles have distinct bases mod 16 of one parity reads
+// 32 distinct banks at each instruction.  The visit order that arranges this
+// buckets the block's particles by (3 rx + Sy ry) mod 16 (the base mod 16; Sz
+// is 16 mod 32) and deals them out one bucket of each parity class at a time:
+// position of rank g in bucket j of class c = (sum over the class of
+// min(size, g)) + (buckets j' < j of the class with size > g), after the even
+// class.  Measured at N = 1e6 (128^3, 3^3-cell blocks): 0.151 -> 0.136 ms with
+// a plane stride of 0 (mod 32) and a 64-bank ordering (mod 32 keys).
+__device__ __forceinline__ void gather_body_f32(const GatherArgs &P, const Blk &bk, unsigned char *eraw, double *red,
+                                                int *run_start, int *run_off, int *bucket) {
+    float *eb = (float *)eraw;
+    const float *emesh = (const float *)P.emesh;
+    const Geom &g = P.g;
+    const int B0 = P.br.Bg[0], B1 = P.br.Bg[1], B2 = P.br.Bg[2];
+    const int Sy = 3 * B0, Sz = gather_plane_f32(B0, B1);
+    const PMBlock blk = pm_block(g, P.br.sb_gather, bk.x);
+    const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nb = B0 * B1 * B2;
+    const Split3 sp3 = split3_init(B0, B1);
+    // stage the brick up to twelve nodes a thread at a time, the loads
+    // unconditional (indices past the brick clamped to its last node) so that
+    // all twelve are in flight together: one node a trip waited ~12 memory
+    // latencies a block at 23^3 nodes, with nothing to overlap them (one
+    // workgroup a CU)
+    constexpr int kSU = 12;
+    for (int t0 = tid; t0 < nb; t0 += kSU * nt) {
+        float v[kSU][3];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+            const int t = min(t0 + u * nt, nb - 1);
+            int tx, ty, tz;
+            split3(sp3, t, tx, ty, tz);
+            const size_t node =
+                ((size_t)wrap3(o2 + tz, g.M[2]) * g.M[1] + wrap3(o1 + ty, g.M[1])) * g.M[0] + wrap3(o0 + tx, g.M[0]);
+            v[u][0] = emesh[3 * node + 0];
+            v[u][1] = emesh[3 * node + 1];
+            v[u][2] = emesh[3 * node + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+            const int t = t0 + u * nt;
+            if (t < nb) {
+                int tx, ty, tz;
+                split3(sp3, t, tx, ty, tz);
+                float *d = eb + 3 * tx + Sy * ty + Sz * tz;
+                d[0] = v[u][0];
+                d[1] = v[u][1];
+                d[2] = v[u][2];
+            }
+        }
+    }
+    const int nrun = pm_runs(g, P.cell_start, blk, run_start, run_off);  // ends with a barrier
+    const int npart = run_off[nrun];
+    // visit order: ord[slot] = run << 10 | offset in the run of the particle
+    // to visit (nrun <= 64; a block with a run of 1024 or more keeps the
+    // sorted order)
+    unsigned short *ord = (unsigned short *)(eraw + 4 * (size_t)Sz * B2);
+    bool reorder = npart <= P.br.ord_cap;  // ord_cap <= 4 blockDim: four particles a thread
+    int kr[4], ko[4];
+    if (reorder) {
+        if (tid < 17) bucket[tid] = 0;  // [16]: set when an offset does not fit 10 bits
+        __syncthreads();
+        int run = 0;
+        double px4[4], py4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // the four position loads first
+            const int k = tid + q * nt;
+            ko[q] = -1;
+            if (k < npart) {
+                const double4 p = P.spos[pm_particle_seq(run_start, run_off, nrun, k, run)];
+                px4[q] = p.x;
+                py4[q] = p.y;
+                const int off = k - run_off[run];
+                ko[q] = (run << 10) | off;
+                if (off >= 1024) bucket[16] = 1;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            kr[q] = -1;
+            if (ko[q] >= 0) {
+                const int rx = (int)floor(px4[q] * g.ih[0]) - o0, ry = (int)floor(py4[q] * g.ih[1]) - o1;
+                const int key = (3 * rx + Sy * ry) & 15;
+                kr[q] = (key << 12) | atomicAdd(&bucket[key], 1);
+            }
+        }
+        __syncthreads();
+        reorder = bucket[16] == 0;
+    }
+    if (reorder) {
+        int neven = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) neven += bucket[2 * j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (kr[q] < 0) continue;
+            const int key = kr[q] >> 12, rank = kr[q] & 4095, par = key & 1, jb = key >> 1;
+            int pos = par ? neven : 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int sz = bucket[2 * j + par];
+                pos += min(sz, rank) + ((j < jb && sz > rank) ? 1 : 0);
+            }
+            ord[pos] = (unsigned short)ko[q];
+        }
+        __syncthreads();
+    }
+    double ksum = 0.0;
+    int run_nat = 0;
+    const int sub = tid % kGatherLanes;
+    const int xh = sub & 1, zh = sub >> 1;
+    const int per_pass = nt / kGatherLanes;
+    const int npass = (npart + per_pass - 1) / per_pass;
+    const GatherOut &o = P.o;
+    const bool want_pp = o.include_pp || o.acc_pp;
+    // sorted index of the particle at a slot of the visit order
+    auto locate = [&](int slot) -> int {
+        if (reorder) {
+            const int v = ord[slot];
+            return run_start[v >> 10] + (v & 1023);
+        }
+        return pm_particle_seq(run_start, run_off, nrun, slot, run_nat);
+    };
+    // each pass's position is loaded one pass ahead, and its epilogue operands
+    // (caller index, PP acceleration) before the field sums, so the memory
+    // latencies overlap the LDS work.  The loads are unconditional (slots past
+    // the block's end read its last particle and discard it): the compiler
+    // then waits for each by its exact count of later loads, where loads
+    // under branches made it wait for all of them
+    const int last = npart - 1;
+    int slot = tid / kGatherLanes;
+    int s = 0;
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (npart > 0) {
+        s = locate(min(slot, last));
+        p = P.spos[s];
+    }
+    for (int pass = 0; pass < npass; ++pass) {
+        const bool live = slot < npart;
+        const int ic = P.perm[s];
+        const double pp = P.acc_pp_s[3 * (size_t)s + (sub < 3 ? sub : 2)];
+        const int slot_n = slot + per_pass;
+        const int s_n = locate(min(slot_n, last));
+        const double4 p_n = P.spos[s_n];
+        float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+        if (live) {
+            float fx, fy, fz;
+            const int ix = base_and_offset<float>(p.x, g.ih[0], fx);
+            const int iy = base_and_offset<float>(p.y, g.ih[1], fy);
+            const int iz = base_and_offset<float>(p.z, g.ih[2], fz);
+            // x: nodes 0-2 take w0, w1, w2 at x; nodes 3-5 take w2, w1, w0 at -x
+            const float sx = xh ? -fx : fx, sz = zh ? -fz : fz;
+            const float a0 = w6_0<float>(sx), a1 = w6_1<float>(sx), a2 = w6_2<float>(sx);
+            const float wx0 = xh ? a2 : a0, wx2 = xh ? a0 : a2;
+            float wy[6];
+            weights6<float>(fy, wy);
+            // z: planes 0, 2, 4 take w0, w2 at z and w1 at -z; planes 5, 3, 1 the same at -z
+            const float wz[3] = {w6_0<float>(sz), w6_2<float>(sz), w6_1<float>(-sz)};
+            const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
+            const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+            if (inside) {
+                const float *base = eb + 3 * (rx + 3 * xh) + Sy * ry + Sz * rz;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const float *plane = base + Sz * (zh ? 5 - 2 * m : 2 * m);
+                    // the plane's six rows summed per x node (weights wy), then
+                    // the three nodes (weights wx): 9 fewer FMAs a plane than
+                    // row by row
+                    float cs[9];
+#pragma unroll
+                    for (int u = 0; u < 9; ++u) cs[u] = wy[0] * plane[u];
+#pragma unroll
+                    for (int b = 1; b < 6; ++b) {
+                        const float *row = plane + Sy * b;
+#pragma unroll
+                        for (int u = 0; u < 9; ++u) cs[u] = fmaf(wy[b], row[u], cs[u]);
+                    }
+                    const float px = fmaf(wx2, cs[6], fmaf(a1, cs[3], wx0 * cs[0]));
+                    const float py = fmaf(wx2, cs[7], fmaf(a1, cs[4], wx0 * cs[1]));
+                    const float pz = fmaf(wx2, cs[8], fmaf(a1, cs[5], wx0 * cs[2]));
+                    ex = fmaf(wz[m], px, ex);
+                    ey = fmaf(wz[m], py, ey);
+                    ez = fmaf(wz[m], pz, ez);
+                }
+            } else {
+                const float wxl[3] = {wx0, a1, wx2};
+                for (int m = 0; m < 3; ++m) {
+                    const int gz = wrap3(iz + (zh ? 5 - 2 * m : 2 * m), g.M[2]);
+                    for (int b = 0; b < 6; ++b) {
+                        const int gy = wrap3(iy + b, g.M[1]);
+                        const float *row = emesh + 3 * (((size_t)gz * g.M[1] + gy) * g.M[0]);
+                        const float wzy = wz[m] * wy[b];
+                        for (int c = 0; c < 3; ++c) {
+                            const int gx = wrap3(ix + 3 * xh + c, g.M[0]);
+                            const float ww = wzy * wxl[c];
+                            ex += row[3 * gx + 0] * ww;
+                            ey += row[3 * gx + 1] * ww;
+                            ez += row[3 * gx + 2] * ww;
+                        }
+                    }
+                }
+            }
+        }
+        // sum the four lanes of each particle
+        ex = quad_sum(ex);
+        ey = quad_sum(ey);
+        ez = quad_sum(ez);
+        if (live && sub < 3) ksum += gather_out_c<float>(P, sub, ic, p.w, want_pp ? pp : 0.0, ex, ey, ez);
+        slot = slot_n;
+        s = s_n;
+        p = p_n;
+    }
+    if (P.o.vel) {
+        ksum = wave_sum(ksum);
+        if ((tid & 63) == 0) red[tid >> 6] = ksum;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kGatherThreads / 64; ++w) t += red[w];
+            P.part[bk.x] = t;
+        }
+    }
+}
+
 template <bool LDS, class T>
 __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
     extern __shared__ __align__(16) unsigned char eraw[];
     __shared__ double red[kGatherThreads / 64];
     __shared__ int run_start[64], run_off[65];
-    gather_body<LDS, T>(P, blk_plain(), eraw, red, run_start, run_off);
+    if constexpr (LDS && sizeof(T) == 4) {
+        __shared__ int bucket[17];
+        gather_body_f32(P, blk_plain(), eraw, red, run_start, run_off, bucket);
+    } else {
+        gather_body<LDS, T>(P, blk_plain(), eraw, red, run_start, run_off);
+    }
 }
 template <class T>
 __global__ __launch_bounds__(kGatherThreads) void k_gather_seg(const GatherArgs *__restrict__ tab, SegGrid S) {
@@ -578,8 +851,16 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_seg(const GatherArgs 
     __shared__ int run_start[64], run_off[65];
     Blk b;
     const int r = seg_locate(S, b);
-    if (tab[r].br.lds_gather) gather_body<true, T>(tab[r], b, eraw, red, run_start, run_off);
-    else gather_body<false, T>(tab[r], b, eraw, red, run_start, run_off);
+    if (tab[r].br.lds_gather) {
+        if constexpr (sizeof(T) == 4) {
+            __shared__ int bucket[17];
+            gather_body_f32(tab[r], b, eraw, red, run_start, run_off, bucket);
+        } else {
+            gather_body<true, T>(tab[r], b, eraw, red, run_start, run_off);
+        }
+    } else {
+        gather_body<false, T>(tab[r], b, eraw, red, run_start, run_off);
+    }
 }
 
 static GatherArgs gather_args(mdf_p3m_ctx *c, const GatherOut &o) {
@@ -603,7 +884,7 @@ int pm_gather(mdf_p3m_ctx *c, int n, const GatherOut &o, hipStream_t s) {
     const bool mixed = c->prm.precision == MDF_PREC_MIXED;
     const dim3 grid(c->brick.nblock_gather);
     if (c->brick.lds_gather) {
-        const size_t lds = 3 * (mixed ? 4 : 8) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2];
+        const size_t lds = gather_lds_bytes(c->brick, mixed);
         if (mixed) {
             MDF_TRY(allow_lds(k_gather<true, float>, lds));
             hipLaunchKernelGGL((k_gather<true, float>), grid, dim3(kGatherThreads), lds, s, P);
@@ -670,7 +951,7 @@ static int build_gather(mdf_p3m_ctx *c, const StepIO &io, void *args, int4 &dims
     o.kick_dt = io.dt;
     *(GatherArgs *)args = gather_args(c, o);
     dims = make_int4(c->brick.nblock_gather, 1, 1, 0);
-    lds = c->brick.lds_gather ? 3 * sizeof(T) * (size_t)c->brick.Bg[0] * c->brick.Bg[1] * c->brick.Bg[2] : 0;
+    lds = c->brick.lds_gather ? gather_lds_bytes(c->brick, sizeof(T) == 4) : 0;
     return MDF_OK;
 }
 template <class T>
