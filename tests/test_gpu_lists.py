@@ -62,7 +62,11 @@ def test_md_run_with_lists_matches_list_free_forces(P3M, system, precision, skin
     pos, vel = dev(S["pos"]), dev(S["vel"])
     acc, _ = e.force(pos)
     acc = acc.clone()
-    tol = 1e-12 if precision == "fp64" else 2e-6
+    # FP32: each path is ~1.1e-6 (RMS, relative) from the FP64 force on this
+    # system, and the list path rounds in the frozen order (its pair sums and
+    # its charge deposits' dithered rounding differ from the sorted order's),
+    # so the two differ by 1.85e-6 - 2.0e-6 from run to run
+    tol = 1e-12 if precision == "fp64" else 3e-6
     total_list_steps = 0
     for chunk in range(3):
         en, _ = e.md_run(pos, vel, acc, 7, 0.02)
