@@ -491,23 +491,40 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             atomicAdd(&P.stats[0], 1);
             if (build) atomicOr(&P.vl_flags[1], 1);  // lists need the whole tile in one pass
         }
-        // 3. stage rows [r0, r1) with their image shifts
-        for (int r = r0 + wave; r < r1; r += NW) {
-            const int tys = r % 6, tzs = r / 6;
-            const double sy = image_shift(2 * by - 2 + tys, g.nsub[1], g.L[1]);
-            const double sz = image_shift(2 * bz - 2 + tzs, g.nsub[2], g.L[2]);
-            int txs = 0;
-            while (txs < 6) {
-                const double sx = image_shift(2 * bx - 2 + txs, g.nsub[0], g.L[0]);
-                int txe = txs + 1;
-                while (txe < 6 && image_shift(2 * bx - 2 + txe, g.nsub[0], g.L[0]) == sx) ++txe;
-                // sub-cells txs .. txe-1 of the row are consecutive cells, so
-                // consecutive in the sorted array: start and length from LDS
-                const int gs = tstart[6 * r + txs];
-                const int ge = gs + (cum[6 * r + txe] - cum[6 * r + txs]);
-                const int dst = cum[6 * r + txs] - base;
-                for (int t = lane; t < ge - gs; t += 64) {
-                    const double4 p = P.spos[gs + t];
+        // 3. stage rows [r0, r1) with their image shifts: each thread takes up
+        // to three tile slots and issues their loads together (indices past the
+        // pass clamped to its last slot, the loads unconditional), so a block
+        // waits one or two memory latencies for its tile (four slots a thread
+        // made the 12-wave FP32 kernel spill); row by row, a wave waited
+        // one for each of its rows.  A slot's sub-cell comes from a binary
+        // search of the sub-cell starts in LDS.
+        {
+            const int nst = cum[6 * r1] - base;
+            constexpr int kSt = 3;
+            for (int t0 = tid; t0 < nst; t0 += kSt * (int)blockDim.x) {
+                double4 pv[kSt];
+                int cv[kSt];
+#pragma unroll
+                for (int u = 0; u < kSt; ++u) {
+                    const int x = min(t0 + u * (int)blockDim.x, nst - 1) + base;
+                    int lo = 6 * r0, hi = 6 * r1;  // cum[lo] <= x < cum[hi]
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (cum[mid] <= x) lo = mid;
+                        else hi = mid;
+                    }
+                    cv[u] = lo;
+                    pv[u] = P.spos[tstart[lo] + (x - cum[lo])];
+                }
+#pragma unroll
+                for (int u = 0; u < kSt; ++u) {
+                    const int t = t0 + u * (int)blockDim.x;
+                    if (t >= nst) continue;
+                    const int c = cv[u], r = c / 6, txs = c - 6 * r, tys = r % 6, tzs = r / 6;
+                    const double sx = image_shift(2 * bx - 2 + txs, g.nsub[0], g.L[0]);
+                    const double sy = image_shift(2 * by - 2 + tys, g.nsub[1], g.L[1]);
+                    const double sz = image_shift(2 * bz - 2 + tzs, g.nsub[2], g.L[2]);
+                    const double4 p = pv[u];
                     if (kRel) {
                         const double cx = ((p.x + sx) - ox) * qs, cy = ((p.y + sy) - oy) * qs,
                                      cz = ((p.z + sz) - oz) * qs;
@@ -515,19 +532,18 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                         const float lx = (float)(cx - (double)hx), ly = (float)(cy - (double)hy),
                                     lz = (float)(cz - (double)hz);
                         if (kPack) {
-                            th4[dst + t] = make_float4(hx, hy, hz, lx);
-                            tl2[dst + t] = make_float2(ly, lz);
+                            th4[t] = make_float4(hx, hy, hz, lx);
+                            tl2[t] = make_float2(ly, lz);
                         } else {
-                            th4[dst + t] = make_float4(hx, hy, hz, 0.0f);
-                            tl4[dst + t] = make_float4(lx, ly, lz, 0.0f);
+                            th4[t] = make_float4(hx, hy, hz, 0.0f);
+                            tl4[t] = make_float4(lx, ly, lz, 0.0f);
                         }
                     } else {
-                        tx[dst + t] = (R)(p.x + sx);
-                        ty[dst + t] = (R)(p.y + sy);
-                        tz[dst + t] = (R)(p.z + sz);
+                        tx[t] = (R)(p.x + sx);
+                        ty[t] = (R)(p.y + sy);
+                        tz[t] = (R)(p.z + sz);
                     }
                 }
-                txs = txe;
             }
         }
         // plane ranges of this pass: home sub-cell h needs tile x sub-cells
