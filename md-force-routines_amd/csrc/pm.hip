@@ -179,9 +179,10 @@ __device__ __forceinline__ T quad_sum(T v) {
     return v;
 }
 
-// A PM block is sb[0] x sb[1] x sb[2] sub-cells (fewer at the upper box
-// faces): whole reference cells (sb = 2 cb), or for the gather, whose FP64
-// brick of one cell can exceed the LDS, half a cell along x (and y).  Its
+// A PM block is at most sb[0] x sb[1] x sb[2] sub-cells (the blocks of an
+// axis share its sub-cells evenly): whole reference cells (sb = 2 cb), or for
+// the gather, whose FP64 brick of one cell can exceed the LDS, half a cell
+// along x (and y).  Its
 // particles are sb[1] sb[2] (or fewer) contiguous runs of the sorted array:
 // one per row of sub-cells along y and z, spanning the block in x.
 struct PMBlock {
@@ -193,10 +194,15 @@ __device__ __forceinline__ PMBlock pm_block(const Geom &g, const int *sb, int bi
     PMBlock b;
     const int nbx = (g.nsub[0] + sb[0] - 1) / sb[0], nby = (g.nsub[1] + sb[1] - 1) / sb[1];
     const int id[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
+    const int nbz = (g.nsub[2] + sb[2] - 1) / sb[2];
+    const int nbd[3] = {nbx, nby, nbz};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        b.s0[d] = id[d] * sb[d];
-        b.ns[d] = min(sb[d], g.nsub[d] - b.s0[d]);
+        // the nb = ceil(n / sb) blocks of an axis split its n sub-cells as
+        // evenly as whole sub-cells allow (at most sb each): a trailing block
+        // of the remainder alone made the workgroups' work uneven
+        b.s0[d] = id[d] * g.nsub[d] / nbd[d];
+        b.ns[d] = (id[d] + 1) * g.nsub[d] / nbd[d] - b.s0[d];
         // the lowest base index of the block's particles; one whose x/h
         // rounds below it (within an ulp of the block face) is outside the
         // brick and takes the kernels' global-memory path
