@@ -1073,6 +1073,7 @@ def main():
             return time.perf_counter() - t0
 
         nthr = c_oracle.lib_omp(a.cpu_threads)[1]
+        print(f"[bench] CPU baseline: {a.cpu_steps} steps on {nthr} threads", file=sys.stderr, flush=True)
         tc = cpu_run(a.cpu_steps, a.cpu_threads)
         cpu = {"value": N * a.cpu_steps / tc, "unit": "particle-steps/s", "cores": nthr, "kind": "port",
                "sample": f"{a.cpu_steps} NVE steps of the same N={N}, {M}^3 workload from the {state}, the C "
