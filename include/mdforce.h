@@ -184,7 +184,8 @@ int mdf_p3m_config(mdf_p3m_ctx *ctx, int *out4);
  * them by timing).  pp_waves: waves per PP workgroup, 4, 8, 12 or 16;
  * assign_shape, gather_shape: PM block shapes in sub-cells per axis,
  * 1 = 8x8x8, 2 = 6x6x6, 3 = 4x4x4, 4 = 2x2x2, 5 = 1x2x2, 6 = 1x1x2,
- * 7 = 1x1x1, 8 = 6x6x4, 9 = 6x4x4 (assignment: 1..4).  0 leaves a choice to the context (its
+ * 7 = 1x1x1, 8 = 6x6x4, 9 = 6x4x4, 10 = 4x4x4 reading the field mesh from
+ * global memory instead of an LDS brick (assignment: 1..4).  0 leaves a choice to the context (its
  * occupancy and LDS rules).  A choice whose LDS does not fit, or whose PP
  * tile would hold fewer than 0.8 of the mean tile, is refused with
  * MDF_EINVAL and the context is unchanged.  The context is rebuilt in place

@@ -166,8 +166,8 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], do
         set_error("PP waves per workgroup %d: need 4, 8, 12 or 16 (0 = automatic)", req[0]);
         return MDF_EINVAL;
     }
-    if (req[1] < 0 || req[1] > 4 || req[2] < 0 || req[2] > 9) {
-        set_error("PM block shapes %d / %d: need 0..4 (assignment) and 0..9 (gather), 0 = automatic", req[1],
+    if (req[1] < 0 || req[1] > 4 || req[2] < 0 || req[2] > 10) {
+        set_error("PM block shapes %d / %d: need 0..4 (assignment) and 0..10 (gather), 0 = automatic", req[1],
                   req[2]);
         return MDF_EINVAL;
     }
@@ -342,8 +342,10 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], do
         // (8 and 9: 3 x 3 x 2 and 3 x 2 x 2 cells, for the gather where a 3^3-cell
         // brick just misses the LDS: at N = 1e6 on 22 cells a side its
         // 25 x 25 x 24 FP32 brick takes 182 KB)
-        static const int shapes[9][3] = {{8, 8, 8}, {6, 6, 6}, {4, 4, 4}, {2, 2, 2}, {1, 2, 2},
-                                         {1, 1, 2}, {1, 1, 1}, {6, 6, 4}, {6, 4, 4}};
+        // (10: 4 x 4 x 4 sub-cells without the LDS brick: the gather reads the
+        // field mesh through the caches)
+        static const int shapes[10][3] = {{8, 8, 8}, {6, 6, 6}, {4, 4, 4}, {2, 2, 2}, {1, 2, 2},
+                                          {1, 1, 2}, {1, 1, 1}, {6, 6, 4}, {6, 4, 4}, {4, 4, 4}};
         auto extent = [&](const int *sb, int *B) {
             size_t nn = 1;
             for (int d = 0; d < 3; ++d) {
@@ -404,7 +406,10 @@ static int create_ctx(const mdf_p3m_params *p, double skin, const int req[3], do
             while (o < 9 && brick_bytes(shapes[by_size[o]], B) > 156 * 1024) ++o;
             bool fits = o < 9;
             int k = fits ? by_size[o] : 3;
-            if (req[2]) {
+            if (req[2] == 10) {
+                k = 9;
+                fits = false;
+            } else if (req[2]) {
                 k = req[2] - 1;
                 if (brick_bytes(shapes[k], B) > 156 * 1024) {
                     delete c;

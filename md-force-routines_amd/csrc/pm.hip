@@ -621,6 +621,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs &P, const Blk &bk, 
 // min(size, g)) + (buckets j' < j of the class with size > g), after the even
 // class.  Measured at N = 1e6 (128^3, 3^3-cell blocks): 0.151 -> 0.136 ms with
 // a plane stride of 0 (mod 32) and a 64-bank ordering (mod 32 keys).
+// LDS false: no brick; every particle reads its nodes from the field mesh in
+// global memory (through the caches), for fine meshes where a block's brick
+// holds many nodes per particle.
+template <bool LDS>
 __device__ __forceinline__ void gather_body_f32(const GatherArgs &P, const Blk &bk, unsigned char *eraw, double *red,
                                                 int *run_start, int *run_off, int *bucket) {
     float *eb = (float *)eraw;
@@ -631,8 +635,8 @@ __device__ __forceinline__ void gather_body_f32(const GatherArgs &P, const Blk &
     const PMBlock blk = pm_block(g, P.br.sb_gather, bk.x);
     const int o0 = blk.o[0], o1 = blk.o[1], o2 = blk.o[2];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nb = B0 * B1 * B2;
-    const Split3 sp3 = split3_init(B0, B1);
+    const int nb = LDS ? B0 * B1 * B2 : 0;
+    const Split3 sp3 = split3_init(max(B0, 1), max(B1, 1));
     // stage the brick up to twelve nodes a thread at a time, the loads
     // unconditional (indices past the brick clamped to its last node) so that
     // all twelve are in flight together: one node a trip waited ~12 memory
@@ -671,7 +675,7 @@ __device__ __forceinline__ void gather_body_f32(const GatherArgs &P, const Blk &
     // to visit (nrun <= 64; a block with a run of 1024 or more keeps the
     // sorted order)
     unsigned short *ord = (unsigned short *)(eraw + 4 * (size_t)Sz * B2);
-    bool reorder = npart <= P.br.ord_cap;  // ord_cap <= 4 blockDim: four particles a thread
+    bool reorder = LDS && npart <= P.br.ord_cap;  // ord_cap <= 4 blockDim: four particles a thread
     int kr[4], ko[4];
     if (reorder) {
         if (tid < 17) bucket[tid] = 0;  // [16]: set when an offset does not fit 10 bits
@@ -773,7 +777,8 @@ __device__ __forceinline__ void gather_body_f32(const GatherArgs &P, const Blk &
             // z: planes 0, 2, 4 take w0, w2 at z and w1 at -z; planes 5, 3, 1 the same at -z
             const float wz[3] = {w6_0<float>(sz), w6_2<float>(sz), w6_1<float>(-sz)};
             const int rx = ix - o0, ry = iy - o1, rz = iz - o2;
-            const bool inside = rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
+            const bool inside =
+                LDS && rx >= 0 && ry >= 0 && rz >= 0 && rx + 5 < B0 && ry + 5 < B1 && rz + 5 < B2;
             if (inside) {
                 const float *base = eb + 3 * (rx + 3 * xh) + Sy * ry + Sz * rz;
 #pragma unroll
@@ -843,9 +848,9 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs P) {
     extern __shared__ __align__(16) unsigned char eraw[];
     __shared__ double red[kGatherThreads / 64];
     __shared__ int run_start[64], run_off[65];
-    if constexpr (LDS && sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4) {
         __shared__ int bucket[17];
-        gather_body_f32(P, blk_plain(), eraw, red, run_start, run_off, bucket);
+        gather_body_f32<LDS>(P, blk_plain(), eraw, red, run_start, run_off, bucket);
     } else {
         gather_body<LDS, T>(P, blk_plain(), eraw, red, run_start, run_off);
     }
@@ -860,12 +865,17 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_seg(const GatherArgs 
     if (tab[r].br.lds_gather) {
         if constexpr (sizeof(T) == 4) {
             __shared__ int bucket[17];
-            gather_body_f32(tab[r], b, eraw, red, run_start, run_off, bucket);
+            gather_body_f32<true>(tab[r], b, eraw, red, run_start, run_off, bucket);
         } else {
             gather_body<true, T>(tab[r], b, eraw, red, run_start, run_off);
         }
     } else {
-        gather_body<false, T>(tab[r], b, eraw, red, run_start, run_off);
+        if constexpr (sizeof(T) == 4) {
+            __shared__ int bucket[17];
+            gather_body_f32<false>(tab[r], b, eraw, red, run_start, run_off, bucket);
+        } else {
+            gather_body<false, T>(tab[r], b, eraw, red, run_start, run_off);
+        }
     }
 }
 

@@ -248,7 +248,7 @@ def tune_launch(eng, pos, vel, dt, steps=20, skins=None, min_gain=0.01, log=None
     launch = lambda c: eng.set_launch(c["pp_waves"], c["assign_shape"], c["gather_shape"])
     try_knob("pp_waves", (4, 8, 12, 16), launch)
     try_knob("assign_shape", (1, 2, 3, 4), launch)
-    try_knob("gather_shape", (1, 2, 8, 9, 3, 4, 5, 6, 7), launch)
+    try_knob("gather_shape", (1, 2, 8, 9, 3, 4, 5, 6, 7, 10), launch)
     cfg = eng.launch_config()
     return {"skin": cur["skin"], "requested": dict(cur), "in_use": cfg, "ms_per_step": best,
             "timings": [(w, ms) for w, ms in timings]}

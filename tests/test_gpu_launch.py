@@ -46,7 +46,7 @@ def test_every_launch_choice_gives_the_same_forces(system, precision):
     tol = 1e-12 if precision == "fp64" else 2e-6
     tried = []
     for waves in (4, 8, 12, 16):
-        for ga in ((0, 0), (4, 3), (3, 5), (2, 7), (1, 1), (3, 8), (4, 9)):
+        for ga in ((0, 0), (4, 3), (3, 5), (2, 7), (1, 1), (3, 8), (4, 9), (3, 10)):
             try:
                 e.set_launch(waves, *ga)
             except MdforceError as err:
@@ -66,7 +66,7 @@ def test_every_launch_choice_gives_the_same_forces(system, precision):
     assert sum(1 for t in tried if t[2] is not None) >= 12
     # refused choices leave the context unchanged
     e.set_launch(0, 0, 0)
-    for bad in ((6, 0, 0), (0, 5, 0), (0, 0, 10), (-4, 0, 0)):
+    for bad in ((6, 0, 0), (0, 5, 0), (0, 0, 11), (-4, 0, 0)):
         with pytest.raises(MdforceError):
             e.set_launch(*bad)
     assert e.launch_config() == auto
