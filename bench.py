@@ -97,6 +97,8 @@ def parse():
     ap.add_argument("--streams", type=int, default=16, help="worker streams of the replica graph")
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both",
                     help="forms to time; value is the chosen form's unless only plain runs")
+    ap.add_argument("--pp-waves", type=int, default=0,
+                    help="PP waves a workgroup without --autotune (4, 8, 12, 16; 0 = the context's choice)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -951,8 +953,11 @@ def main():
             tl = autotune.tune_launch(eng, start_pos, start_vel, a.dt, steps=20)
             print(f"[bench] {precision}: launch tuned to {tl['requested']} ({tl['ms_per_step']:.4f} ms/step in "
                   f"{len(tl['timings'])} timings)", file=sys.stderr, flush=True)
-        elif a.skin > 0:
-            eng.set_skin(a.skin)
+        else:
+            if a.skin > 0:
+                eng.set_skin(a.skin)
+            if a.pp_waves:
+                eng.set_launch(a.pp_waves)
         skin = tl["skin"] if tl else a.skin
         pp_cfg = eng.config()
         pos = start_pos.clone()
