@@ -344,7 +344,14 @@ __device__ __forceinline__ float half_wave_sum(float v) {
 
 // Slots a list of cnt entries occupies with its padding: whole chunks of
 // 2 kGL entries and at least two padding slots (one list word) at the end.
-__device__ __forceinline__ int list_padded(int cnt) { return (cnt + 2 + 2 * kGL - 1) & ~(2 * kGL - 1); }
+// List layout: chunks of 2 kGL entries; dword l of a chunk holds its entries
+// l (low half) and kGL + l (high half), so the first kGL entries of a chunk
+// are the low halves of all its dwords.  Entry e sits at 16-bit slot
+// list_pos(e); a list is padded with the dummy slot to a whole chunk.
+__device__ __forceinline__ int list_pos(int e) {
+    return (e & ~(2 * kGL - 1)) + 2 * (e & (kGL - 1)) + ((e / kGL) & 1);
+}
+__device__ __forceinline__ int list_padded(int cnt) { return (cnt + 2 * kGL - 1) & ~(2 * kGL - 1); }
 
 // NW waves per workgroup share the tile: 4 for tiles up to 64 KB, 16 for
 // larger ones (large rc), where one workgroup per CU would otherwise leave a
@@ -653,25 +660,29 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             // build: the queued j of this half go to its i's list, slots lc .. lc + cnt - 1
             // (i itself, queued by the FP32 sweep, as the dummy slot)
             auto list_put = [&](int jj, int at) {
-                if (at < P.vl_cap) lst[at] = (unsigned short)(jj == islot ? kFar : jj);
+                if (at < P.vl_cap) lst[list_pos(at)] = (unsigned short)(jj == islot ? kFar : jj);
             };
             if (use_list) {
                 // the step's pairs from i's list: every j within the list radius at
-                // the last build, evaluated with the exact cutoff.  A lane takes two
-                // consecutive entries (one dword) of each chunk of 2 kGL; the
-                // chunks are loaded two ahead of their use, so that the list reads
-                // from memory overlap the pair terms of the chunks before
-                // a list is padded with the dummy slot to a whole chunk past its
-                // last entry, so its last chunk word holds padding only; a lane
-                // past i's padded end re-reads that word (the load index is
-                // clamped: the load is unconditional, and its wait sits where
-                // the value is used)
+                // the last build, evaluated with the exact cutoff.  A wave runs
+                // the half-chunks (kGL entries) of the longer of its two lists:
+                // whole chunks (two entries a lane, one dword) and, for an odd
+                // count, a last half-chunk (the low entries alone); padding to
+                // whole half-chunks, not chunks, saves ~16 entries a list.  A
+                // lane's chunks past its own list take the dummy pair (a select;
+                // the load index is clamped to the list's last dword, the load
+                // unconditional, its wait where the value is used)
                 const int cnt = have_i ? P.vl_cnt[gi] : 0;
-                const int cntp = list_padded(cnt);
-                const int nchunk = max(readlane_i(cntp, 0), readlane_i(cntp, kGL)) / (2 * kGL);
+                const int hch = (cnt + kGL - 1) / kGL;  // this half's half-chunks
+                const int hmax = max(readlane_i(hch, 0), readlane_i(hch, kGL));
+                const int nchunk = hmax >> 1;      // whole chunks of the wave
+                const bool tail = (hmax & 1) != 0;  // and a last half-chunk
+                const int ownc = (hch + 1) >> 1;   // chunks of this half's list in memory
                 const unsigned *lw = (const unsigned *)lst;  // vl_cap is a multiple of 64: dword aligned
-                const int wlast = (cntp >> 1) - 1;
+                const int wlast = max(ownc * kGL - 1, 0);
+                const unsigned kpad = (unsigned)kFar | ((unsigned)kFar << 16);
                 auto chunk_word = [&](int c) -> unsigned { return lw[min(c * kGL + gl, wlast)]; };
+                auto own_word = [&](int c, unsigned w) -> unsigned { return c < ownc ? w : kpad; };
                 // FP32: the pair terms come from the table alone (s' clamped to
                 // s_lo at the low end); close pairs (s' < s_lo, the release of
                 // a random start) are corrected in a second pass over the list,
@@ -703,17 +714,31 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
                 for (int c = 0; c < KPF; ++c) wv[c] = chunk_word(c);
 #pragma unroll
                 for (int c = 0; c < KPF; ++c)
-                    if (c < nchunk) chunk_pairs(wv[c]);
-                for (int c = KPF; c < nchunk; ++c) chunk_pairs(chunk_word(c));
+                    if (c < nchunk) chunk_pairs(own_word(c, wv[c]));
+                for (int c = KPF; c < nchunk; ++c) chunk_pairs(own_word(c, chunk_word(c)));
+                if (tail) {  // the low entries of chunk nchunk
+                    const unsigned w = own_word(nchunk, chunk_word(nchunk));
+                    R dx, dy, dz;
+                    disp((int)(w & 0xffffu), dx, dy, dz);
+                    if constexpr (kRel) {
+                        float s0;
+                        pp_pair_tab(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u, s0);
+                        smin = fminf(smin, s0);
+                    } else {
+                        pp_pair_masked<R>(P, tab, true, dx, dy, dz, fx, fy, fz, u);
+                    }
+                }
                 if constexpr (kRel) {
                     if (__builtin_amdgcn_ballot_w64(smin < P.s_lo)) {  // rare: close pairs in some list
-                        for (int c = 0; c < nchunk; ++c) {
-                            const unsigned w = chunk_word(c);
+                        for (int c = 0; c < nchunk + (tail ? 1 : 0); ++c) {
+                            const unsigned w = own_word(c, chunk_word(c));
                             R dx, dy, dz;
                             disp((int)(w & 0xffffu), dx, dy, dz);
                             pp_pair_fix(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u);
-                            disp((int)(w >> 16), dx, dy, dz);
-                            pp_pair_fix(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u);
+                            if (c < nchunk) {
+                                disp((int)(w >> 16), dx, dy, dz);
+                                pp_pair_fix(P, tbase, s_top_v, dx, dy, dz, fx, fy, fz, u);
+                            }
                         }
                     }
                 }
@@ -818,9 +843,9 @@ __device__ __forceinline__ void pp_body(const PPArgs<R> &P, const Blk &blk, unsi
             if (build && have_i) {
                 const int total = (grp ? lc1 : lc0) + qmine;
                 const int pend = list_padded(total);
-                // dummy padding of the list's last chunk (two slots a lane)
+                // dummy padding to the end of the last chunk (at most two slots a lane)
                 if (pend <= P.vl_cap) {
-                    for (int at = total + gl; at < pend; at += kGL) lst[at] = (unsigned short)kFar;
+                    for (int at = total + gl; at < pend; at += kGL) lst[list_pos(at)] = (unsigned short)kFar;
                 }
                 if (gl == 0) {
                     P.vl_cnt[gi] = min(total, P.vl_cap);
