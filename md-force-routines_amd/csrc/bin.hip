@@ -49,6 +49,7 @@ struct BinArgs {
     double4 *spos;
     int *perm;
     const int *mode;   // pair-list mode of the step (2: lists, binning skipped), or null
+    double4 *vl_ref;   // pair lists' build positions, written beside spos on a build, or null
 };
 
 __device__ __forceinline__ void bin_count_body(const BinArgs &A, const Blk &b) {
@@ -185,8 +186,10 @@ __device__ __forceinline__ void bin_place_body(const BinArgs &A, const Blk &b) {
     const double x = wrap_box(pos[3 * i + 0], g.L[0]);
     const double y = wrap_box(pos[3 * i + 1], g.L[1]);
     const double z = wrap_box(pos[3 * i + 2], g.L[2]);
-    spos[s] = make_double4(x, y, z, Z ? Z[i] : 1.0);
+    const double4 v = make_double4(x, y, z, Z ? Z[i] : 1.0);
+    spos[s] = v;
     perm[s] = i;
+    if (A.vl_ref) A.vl_ref[s] = v;
 }
 
 __global__ void k_bin_place(BinArgs A) { bin_place_body(A, blk_plain()); }
@@ -242,13 +245,16 @@ static BinArgs bin_args(mdf_p3m_ctx *c, const double *pos, const double *Z, int 
     A.spos = c->spos;
     A.perm = c->perm;
     A.mode = nullptr;
+    A.vl_ref = nullptr;
     return A;
 }
 
-int bin_particles_mode(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, const int *mode, hipStream_t s) {
+int bin_particles_mode(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, const int *mode, hipStream_t s,
+                       double4 *vl_ref) {
     if (n <= 0) return MDF_OK;
     BinArgs A = bin_args(c, pos, Z, n);
     A.mode = mode;
+    A.vl_ref = vl_ref;
     const unsigned nb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, s, A);
     MDF_LAUNCHED();

@@ -227,7 +227,9 @@ void prof_mark(mdf_p3m_ctx *c, int stage, hipStream_t s);
 // (with pair lists on, the next list-aware step then builds);
 // bin_particles_mode: skipped on the device when *mode == 2 (a list step).
 int bin_particles(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStream_t s);
-int bin_particles_mode(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, const int *mode, hipStream_t s);
+// vl_ref: the pair lists' build positions, written with the sorted copy (or null)
+int bin_particles_mode(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, const int *mode, hipStream_t s,
+                       double4 *vl_ref = nullptr);
 int pp_forces(mdf_p3m_ctx *c, int n, hipStream_t s);
 int pm_assign(mdf_p3m_ctx *c, int n, double *rho_out, hipStream_t s);
 int pm_solve(mdf_p3m_ctx *c, hipStream_t s);

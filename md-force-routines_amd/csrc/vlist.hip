@@ -25,8 +25,9 @@
 //   k_vl_decide  one block: build (requested, last build incomplete, or a
 //                displacement over s / 2) or lists; the binning and the PP
 //                kernel read the mode
-//   binning      skipped on a list step
-//   k_vl_refresh on a build step: the build positions := the new sorted copy
+//   binning      skipped on a list step; on a build step its placing kernel
+//                also writes the build positions (the new sorted copy)
+// (replica batches run k_vl_refresh as a stage of its own for the last)
 #include "mdf_internal.h"
 
 namespace mdf {
@@ -193,9 +194,9 @@ int bin_step(mdf_p3m_ctx *c, const double *pos, const double *Z, int n, hipStrea
     MDF_LAUNCHED();
     hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(1024), 0, s, VlDecideArgs{c->vl_flags, A.bmax, nb, vl_threshold(c)});
     MDF_LAUNCHED();
-    MDF_TRY(bin_particles_mode(c, pos, Z, n, c->vl_flags + 3, s));
-    hipLaunchKernelGGL(k_vl_refresh, dim3((n + 255) / 256), dim3(256), 0, s, VlRefreshArgs{c->spos, n, c->vl_ref, c->vl_flags});
-    MDF_LAUNCHED();
+    // a build's binning also records the build positions (the list steps'
+    // reference), in its placing kernel: no launch of its own every step
+    MDF_TRY(bin_particles_mode(c, pos, Z, n, c->vl_flags + 3, s, c->vl_ref));
     return MDF_OK;
 }
 
