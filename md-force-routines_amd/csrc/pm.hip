@@ -9,7 +9,7 @@
 // Work is organised per PM block of CB^3 reference cells (CB chosen per
 // context so the block's mesh brick fits LDS; the block's particles are
 // contiguous runs of the sorted array).  Each particle gets a group of
-// lanes: eight for the scatter, six of them taking one z-plane (36 nodes) of
+// lanes: six for the scatter, each taking one z-plane (36 nodes) of
 // its 6x6x6 stencil each; four for the gather, each taking a quarter (FP64:
 // three z-planes by three y-rows of six x nodes; FP32: three z-planes by six
 // y-rows of three x nodes):
@@ -129,7 +129,7 @@ __device__ __forceinline__ int wrap3(int i, int M) {
 // w0, w1, w2 at +-y and of its planes w0, w2 at +-z and w1 at -+z: every lane
 // evaluates the same polynomials, with no lane-dependent selects.
 constexpr int kGatherLanes = 4;   // lanes a particle in the gather
-constexpr int kScatterLanes = 8;  // in the assignment (six busy: one z-plane each)
+constexpr int kScatterLanes = 6;  // in the assignment (one z-plane each)
 
 template <class T>
 struct Quarter {
@@ -322,17 +322,20 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
         // stays out of the charge sum
         inv_s = wscale<T>() / (double)kq;
     }
-    // eight lanes a particle, six of them one stencil z-plane each: with half
-    // as many particles per wave as the four-lane split of the gather, fewer
-    // LDS atomics of one instruction land on the same node (measured ~6%
-    // faster here; the gather, with plain reads, gains from the denser split)
-    const int sub = threadIdx.x % kScatterLanes;
+    // six lanes a particle, one stencil z-plane each, ten particles a wave
+    // (lanes 60-63 idle).  Eight lanes a particle (two idle) measured ~6%
+    // faster than four when the deposits took six instructions, from fewer
+    // LDS atomics of one instruction on one node; with four-instruction
+    // deposits the busy lanes count for more
+    const int lane = threadIdx.x & 63;
+    const int sub = lane % kScatterLanes, pslot = lane / kScatterLanes;
     int run = 0;
     // each particle's position is loaded one trip ahead, so its memory
     // latency overlaps the deposits of the particle before
     // (unconditionally: a trip past the end reads the block's last particle)
-    const int kstep = blockDim.x / kScatterLanes;
-    int k = threadIdx.x / kScatterLanes;
+    constexpr int kPerWave = 64 / kScatterLanes;
+    const int kstep = (int)(blockDim.x >> 6) * kPerWave;
+    int k = (int)(threadIdx.x >> 6) * kPerWave + min(pslot, kPerWave - 1);
     int si_n = 0;
     double4 p_n = make_double4(0.0, 0.0, 0.0, 0.0);
     if (k < npart) {
@@ -344,7 +347,7 @@ __device__ __forceinline__ void assign_body(const AssignArgs &P, const Blk &bk, 
         const double4 p = p_n;
         si_n = pm_particle_seq(run_start, run_off, nrun, min(k + kstep, npart - 1), run);
         p_n = P.spos[si_n];
-        if (sub >= 6) continue;
+        if (pslot >= kPerWave) continue;
         T fx, fy, fz, wx[6], wy[6];
         const int ix = base_and_offset<T>(p.x, g.ih[0], fx);
         const int iy = base_and_offset<T>(p.y, g.ih[1], fy);
