@@ -83,6 +83,8 @@ def parse():
                     help="mesh sizes the replica autotuner considers (comma separated)")
     ap.add_argument("--mlp-epochs", type=int, default=20)
     ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--drift-skin", type=float, default=0.0,
+                    help="pair-list skin of both forms in --workload drift (0 = no lists)")
     ap.add_argument("--eq-steps", type=int, default=300)
     ap.add_argument("--uniform-steps", type=int, default=1000)
     ap.add_argument("--start", choices=("both", "uniform", "equilibrated"), default="both",
@@ -712,6 +714,8 @@ def run_drift(a):
         for prec in ("fp64", "mixed"):
             e = P3M(N, M, L, a.kappa, a.alpha, a.rc, precision=prec, device=dev)
             e.build_green()
+            if a.drift_skin > 0:
+                e.set_skin(a.drift_skin)
             p, v = pos.clone(), vel.clone()
             acc, _ = e.force(p)
             acc = acc.clone()
@@ -754,7 +758,8 @@ def run_drift(a):
         "config": {"workload": f"N={N}, {M}^3, Gamma {a.gamma:g}, kappa {a.kappa:g}, NVE trajectories of both "
                                f"forms from " + ", ".join(f"the {k} start ({v['chosen']['steps']} steps)"
                                                           for k, v in out.items()),
-                   "alpha": a.alpha, "rc": a.rc, "dt": a.dt, "starts": out, "parallelism": "single GPU"},
+                   "alpha": a.alpha, "rc": a.rc, "dt": a.dt, "pair_list_skin": a.drift_skin, "starts": out,
+                   "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)
