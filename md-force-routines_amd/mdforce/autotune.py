@@ -202,12 +202,14 @@ def time_steps(eng, pos, vel, dt, steps=20, warmup=2):
 
 
 def tune_launch(eng, pos, vel, dt, steps=20, skins=None, min_gain=0.01, log=None):
-    """Launch choices of eng by measurement, one knob at a time: the skin
-    (skin_candidates; the cell size follows), then the PP waves per workgroup,
-    then the assignment block shape, then the gather block shape.  A choice
-    replaces the current one only when it is faster by more than min_gain.
-    Choices the context refuses (LDS) are skipped.  eng is left with the
-    chosen settings; returns a dict of them and every timing."""
+    """Launch choices of eng by measurement: the skin (skin_candidates; the
+    cell size follows) and the PP waves per workgroup together (every pair:
+    the best wave count differs between skins, and one knob at a time could
+    settle on a skin whose best waves it never timed), then the assignment
+    block shape, then the gather block shape.  A choice replaces the current
+    one only when it is faster by more than min_gain.  Choices the context
+    refuses (LDS) are skipped.  eng is left with the chosen settings; returns
+    a dict of them and every timing."""
     if skins is None:
         skins = skin_candidates(eng.box, eng.rc)
     timings = []
@@ -244,9 +246,25 @@ def tune_launch(eng, pos, vel, dt, steps=20, skins=None, min_gain=0.01, log=None
                 cur.update(trial)
         apply(cur)
 
-    try_knob("skin", skins, lambda c: eng.set_skin(c["skin"]))
     launch = lambda c: eng.set_launch(c["pp_waves"], c["assign_shape"], c["gather_shape"])
-    try_knob("pp_waves", (4, 8, 12, 16), launch)
+    start = dict(cur)
+    for sk in [start["skin"]] + [x for x in skins if x != start["skin"]]:
+        for w in (4, 8, 12, 16):
+            trial = dict(start, skin=sk, pp_waves=w)
+            try:
+                eng.set_skin(sk)
+                launch(trial)
+            except MdforceError as err:
+                if err.code != MDF_EINVAL:
+                    raise
+                timings.append((trial, None))
+                continue
+            ms = measure(trial)
+            if ms < best * (1.0 - min_gain):
+                best = ms
+                cur.update(trial)
+    eng.set_skin(cur["skin"])
+    launch(cur)
     try_knob("assign_shape", (1, 2, 3, 4), launch)
     try_knob("gather_shape", (1, 2, 8, 9, 3, 4, 5, 6, 7, 10), launch)
     cfg = eng.launch_config()
