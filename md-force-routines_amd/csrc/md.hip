@@ -461,10 +461,15 @@ int density_modes(mdf_p3m_ctx *c, const double *pos, int n, const double *q, int
 // blocks), so that the 2 x kModeChunk wave sums at the end of each wave cost
 // little beside its particles' work.  With one particle in four threads
 // (1024 blocks at N = 1e6) the sums took as long as the sincos and recurrences.
+// A multiple of 8: workgroups are dealt to the 8 XCDs by their linear index
+// x + nb (axis + 3 chunk), so with nb = 0 (mod 8) the three axes' blocks of
+// one particle range run on one XCD and can share its L2 for the positions
+// (each reads one coordinate of the same lines).
 static int modes_blocks(int n) {
     int nb = (n + 4095) / 4096;
     if (nb > 256) nb = 256;
-    return nb < 1 ? 1 : nb;
+    nb = (nb + 7) & ~7;
+    return nb;
 }
 
 int density_modes_grid(mdf_p3m_ctx *c, const double *pos, int n, double dq, int nq, double *out, hipStream_t s) {
